@@ -1,0 +1,309 @@
+/*
+ * meteoros_cloud.h -- C ABI of the MI355X cloud compute stage.
+ *
+ * This header is the drop-in boundary that replaces the Vulkan compute
+ * pipeline interface of Meteoros' cloud pass:
+ *
+ *   set 0  ping-pong result images       Renderer.cpp:924-927, :1034-1086
+ *   set 1  LF/HF sampler3D, curl/weather sampler2D, god-ray image
+ *                                         Renderer.cpp:930-938, :1087-1165
+ *   set 2  CameraUBO (cloud pass)         camera.h:12-18, cloudRayMarch.comp:15-21
+ *   set 3  TimeUBO                        Scene.h:12-21, cloudRayMarch.comp:23-31
+ *   set 4  SunAndSkyUBO                   Sky.h:9-15, cloudRayMarch.comp:33-39
+ *   reprojection set 2 = old CameraUBO    reprojection.comp:21-27
+ *   dispatch order reprojection -> march  Renderer.cpp:653-722
+ *
+ * Plain pointers and sizes only.  Every entry point returns an mtc_status;
+ * no C++ exception crosses the boundary.  A context is not internally
+ * locked: use one context per host thread.
+ */
+#ifndef METEOROS_CLOUD_H
+#define METEOROS_CLOUD_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ------------------------------------------------------------------ */
+/* status codes                                                        */
+/* ------------------------------------------------------------------ */
+typedef enum mtc_status {
+  MTC_OK = 0,
+  MTC_ERR_INVALID_ARG = 1,   /* null pointer, bad size, bad enum           */
+  MTC_ERR_OUT_OF_MEMORY = 2, /* device or host allocation failed           */
+  MTC_ERR_HIP = 3,           /* a HIP runtime call failed                  */
+  MTC_ERR_UNSUPPORTED = 4,   /* valid request this build does not handle   */
+  MTC_ERR_STATE = 5          /* call out of order (e.g. no volumes yet)    */
+} mtc_status;
+
+/* ------------------------------------------------------------------ */
+/* uniform blocks, byte-identical to the reference's std140 UBOs       */
+/* ------------------------------------------------------------------ */
+
+/* CameraUBO (camera.h:12-18).  Matrices are column-major, view[c*4+r].
+ * The shaders use eye = -eye.xyz as the ray origin
+ * (cloudRayMarch.comp:707, reprojection.comp:205). */
+typedef struct mtc_camera {
+  float view[16];        /* offset   0 */
+  float proj[16];        /* offset  64 */
+  float eye[4];          /* offset 128 */
+  float tan_fov_by2[2];  /* offset 144: (aspect*tan(fovy/2), tan(fovy/2)) */
+} mtc_camera;            /* 152 bytes */
+
+/* TimeUBO (Scene.h:12-21, cloudRayMarch.comp:23-31). */
+typedef struct mtc_time {
+  float halton1[4];      /* offset  0: H3(1..4)   */
+  float halton2[4];      /* offset 16: H3(5..8)   */
+  float halton3[4];      /* offset 32: H3(9..12)  */
+  float halton4[4];      /* offset 48: H3(13..16) */
+  float time[2];         /* offset 64: (delta, total) seconds */
+  int32_t frame_count_mod16; /* offset 72 */
+} mtc_time;              /* 76 bytes */
+
+/* SunAndSkyUBO (Sky.h:9-15).  The shipped shader never reads it and uses
+ * #define SUN_LOCATION / SUN_INTENSITY instead (cloudRayMarch.comp:94-99);
+ * here sun_location.xyz is the lighting sun (SUN_LOCATION) and
+ * sun_intensity scales the Preetham background (SUN_INTENSITY). */
+typedef struct mtc_sun {
+  float sun_location[4];  /* offset  0 */
+  float sun_direction[4]; /* offset 16 (carried, unused by the stage) */
+  float light_color[4];   /* offset 32 (carried, unused by the stage) */
+  float sun_intensity;    /* offset 48 */
+} mtc_sun;                /* 52 bytes */
+
+/* ------------------------------------------------------------------ */
+/* runtime configuration: every #define of cloudRayMarch.comp and       */
+/* reprojection.comp that the configurations vary                       */
+/* ------------------------------------------------------------------ */
+enum {
+  MTC_STEPS_REFERENCE = 0, /* floor(mix(steps_min, steps_max, 1-dir.y)), cloudRayMarch.comp:572 */
+  MTC_STEPS_FIXED = 1      /* exactly steps_max as the divisor of the shell span */
+};
+enum {
+  MTC_COVERAGE_CONSTANT = 0, /* cloud_coverage = coverage_constant (cloudRayMarch.comp:529) */
+  MTC_COVERAGE_WEATHER = 1   /* coverage = weather.r, height gradient from weather.g
+                                (the path commented out at cloudRayMarch.comp:515-525) */
+};
+
+typedef struct mtc_cloud_params {
+  /* geometry, metres (cloudRayMarch.comp:75-78) */
+  double earth_radius;        /* EARTH_RADIUS */
+  double shell_inner_radius;  /* ATMOSPHERE_RADIUS_INNER */
+  double shell_outer_radius;  /* ATMOSPHERE_RADIUS_OUTER */
+  /* background sky sun (BACKGROUND_SKY_SUN_LOCATION, :95) */
+  float sky_sun_location[4];
+  /* primary march (:565-688) */
+  int32_t step_mode;          /* MTC_STEPS_* */
+  int32_t steps_min;          /* 35 */
+  int32_t steps_max;          /* 60 */
+  int32_t light_samples;      /* 6 (cone samples, :652), at most 6 */
+  float base_density_factor;  /* 0.38 (:571) */
+  float jitter_factor;        /* 1.18 (:568) */
+  float jitter_dim;           /* 75, the ivec2(75) of :632 */
+  float sample_divisor;       /* 8, :635 */
+  float erosion_scale;        /* 1.4, :645 */
+  float accum_scale;          /* 0.5, :648 */
+  float cone_erosion_scale;   /* 1.5, :665 */
+  float brightness;           /* 5.0, :672 */
+  /* Henyey-Greenstein (:587-590) */
+  float hg_eccentricity;      /* 0.6 */
+  float hg_silver_intensity;  /* 0.7 */
+  float hg_silver_spread;     /* 0.1 */
+  /* wind (:89-91, :489-497) */
+  float wind_direction[4];    /* (1,0,0) */
+  float wind_upward_bias;     /* 0.1 */
+  float cloud_speed;          /* 0.080 sample-space units per second */
+  float cloud_top_offset;     /* 1.0 */
+  float wind_skew;            /* 0.009 */
+  /* coverage (:529) */
+  int32_t coverage_mode;      /* MTC_COVERAGE_* */
+  float coverage_constant;    /* 0.6 */
+  float weather_scale;        /* weather uv = unskewed point.xz * weather_scale */
+  /* composite (:713-716, :762) */
+  float background_min;       /* 0.620 */
+  float cloud_fade_start;     /* 0.06 */
+  float cloud_fade_end;       /* 0.2 */
+  /* reprojection (reprojection.comp:66) */
+  int32_t motion_blur_samples;/* 10 */
+  int32_t reserved[3];
+} mtc_cloud_params;
+
+/* ------------------------------------------------------------------ */
+/* host-side uniform producers (camera.cpp, Scene.cpp, Sky.cpp)        */
+/* ------------------------------------------------------------------ */
+
+/* State of camera.h's Camera class (eye, ref, derived basis, lens). */
+typedef struct mtc_camera_state {
+  float eye[3];
+  float ref[3];
+  float forward[3];
+  float right[3];
+  float up[3];
+  float world_up[3];
+  int32_t width, height;
+  float fovy_deg;
+  float aspect;
+  float near_clip, far_clip;
+} mtc_camera_state;
+
+/* Camera::Camera + RecomputeAttributes (camera.cpp:3-17, :68-77). */
+int mtc_camera_init(mtc_camera_state* cam, const float eye[3], const float ref[3],
+                    int32_t width, int32_t height, float fovy_deg,
+                    float near_clip, float far_clip);
+/* Camera::RotateAboutUp / RotateAboutRight (camera.cpp:79-96). */
+int mtc_camera_rotate_about_up(mtc_camera_state* cam, float deg);
+int mtc_camera_rotate_about_right(mtc_camera_state* cam, float deg);
+/* Camera::TranslateAlong{Look,Right,Up} (camera.cpp:98-117). */
+int mtc_camera_translate(mtc_camera_state* cam, float look, float right, float up);
+/* Camera::UpdateBuffer (camera.cpp:31-42): glm lookAt / perspective (ZO),
+ * y-flipped projection, tanFovBy2 with PI = 3.14159. */
+int mtc_camera_ubo(const mtc_camera_state* cam, mtc_camera* out);
+
+/* Scene::HaltonSequenceAt (Scene.cpp:125-138). */
+float mtc_halton(int32_t index, int32_t base);
+/* Scene::InitializeTime (Scene.cpp:86-119) with delta = total = 0. */
+int mtc_time_init(mtc_time* t);
+/* Scene::UpdateTime (Scene.cpp:65-85) with the wall clock replaced by dt. */
+int mtc_time_advance(mtc_time* t, float dt);
+
+/* Shipped constants of cloudRayMarch.comp / reprojection.comp. */
+int mtc_params_reference(mtc_cloud_params* p, mtc_sun* sun);
+/* The workload presets (1..5) of this stage; see mtc_preset_info. */
+int mtc_params_preset(int32_t preset, mtc_cloud_params* p, mtc_sun* sun);
+/* Sun at (elevation, azimuth) degrees, azimuth 0 = -z, at distance_m. */
+int mtc_sun_from_angles(float elevation_deg, float azimuth_deg, float distance_m,
+                        mtc_sun* sun, mtc_cloud_params* p);
+/* Convert a wind speed in m/s into cloud_speed sample units per second:
+ * one sample unit spans sample_divisor * shell thickness metres. */
+float mtc_wind_speed_units(const mtc_cloud_params* p, float metres_per_second);
+
+/* ------------------------------------------------------------------ */
+/* volumes                                                             */
+/* ------------------------------------------------------------------ */
+enum {
+  MTC_FMT_UNORM8 = 0, /* 4 x uint8 per texel, R8G8B8A8_UNORM (Sky.cpp:31-57) */
+  MTC_FMT_FP16 = 1    /* 4 x IEEE half per texel */
+};
+
+/* Noise and map resources.  Layout of every array: x fastest, then y,
+ * then z, tightly packed, 4 channels per texel
+ * (ImageLoadingUtility.cpp:75-139, Image.cpp:181-212).  Dimensions must
+ * be powers of two (REPEAT wrap is a mask). */
+typedef struct mtc_volume_desc {
+  int32_t lf_dim;       /* low-frequency Perlin-Worley volume, lf_dim^3   */
+  int32_t hf_dim;       /* high-frequency Worley volume, hf_dim^3         */
+  int32_t curl_dim;     /* curl-noise map, curl_dim^2                     */
+  int32_t weather_dim;  /* weather map, weather_dim^2                     */
+  int32_t format;       /* MTC_FMT_* of lf and hf (maps are always UNORM8) */
+} mtc_volume_desc;
+
+/* CPU-identical noise generator settings (kernel a). */
+typedef struct mtc_noise_desc {
+  uint32_t seed_volumes;   /* LF / HF / curl seed */
+  uint32_t seed_weather;   /* weather map seed     */
+} mtc_noise_desc;
+
+/* ------------------------------------------------------------------ */
+/* context                                                             */
+/* ------------------------------------------------------------------ */
+enum {
+  MTC_KERNEL_PLAIN = 0,     /* one lane per ray, FP32, GLSL op order          */
+  MTC_KERNEL_FAST = 1,      /* FP16 texel cache + hoisted cone, FP32 accum    */
+  MTC_KERNEL_PRECISION = 2, /* FP16 volumes, BF16 density math, FP64 shell     */
+  MTC_KERNEL_LEARNED = 3    /* FAST with the cone march replaced by the MLP    */
+};
+
+enum {
+  MTC_FLAG_COUNTERS = 1,     /* accumulate mtc_counters with device atomics  */
+  MTC_FLAG_FULL_MARCH = 2    /* march every pixel (no 4x4 selection)          */
+};
+
+typedef struct mtc_desc {
+  int32_t width;
+  int32_t height;
+  mtc_volume_desc volumes;
+  int32_t kernel;           /* MTC_KERNEL_* */
+  int32_t flags;            /* MTC_FLAG_*   */
+  int32_t device;           /* HIP device ordinal */
+  int32_t reserved;
+} mtc_desc;
+
+/* Per-frame work counters (counting build / MTC_FLAG_COUNTERS). */
+typedef struct mtc_counters {
+  uint64_t rays_marched;     /* rays that entered rayMarch            */
+  uint64_t rays_ocean;       /* dir.y < 0 early-outs                  */
+  uint64_t rays_sky;         /* 0 <= dir.y < fade start early-outs    */
+  uint64_t primary_steps;    /* executed primary iterations (ray-steps) */
+  uint64_t cloud_steps;      /* primary iterations with base density > 0 */
+  uint64_t cone_samples;     /* cone LF samples                       */
+  uint64_t cone_erosions;    /* cone samples with density > 0          */
+  uint64_t early_exits;      /* accumDensity >= 1 breaks              */
+  uint64_t mlp_queries;      /* learned-light queries                 */
+  uint64_t mlp_fallbacks;    /* learned-light queries sent to the cone */
+} mtc_counters;
+
+typedef struct mtc_ctx mtc_ctx;
+
+/* Last error message of this thread (never NULL). */
+const char* mtc_last_error(void);
+/* Library version string. */
+const char* mtc_version(void);
+/* Number of HIP devices visible (0 when none; never an error). */
+int mtc_device_count(int32_t* count);
+
+int mtc_create(const mtc_desc* desc, mtc_ctx** out);
+int mtc_destroy(mtc_ctx* ctx);
+
+/* Upload volumes (Sky::CreateCloudResources, Sky.cpp:25-58).  lf/hf use
+ * desc.volumes.format; curl and weather are UNORM8 RGBA.  Host memory is
+ * only read during the call. */
+int mtc_set_volumes(mtc_ctx* ctx, const void* lf, const void* hf,
+                    const void* curl, const void* weather);
+/* Generate the volumes on the device (kernel a). */
+int mtc_generate_volumes(mtc_ctx* ctx, const mtc_noise_desc* noise);
+/* Copy the resident volumes back to host memory (any pointer may be NULL). */
+int mtc_get_volumes(mtc_ctx* ctx, void* lf, void* hf, void* curl, void* weather);
+
+/* One frame of the cloud compute stage (Renderer.cpp:653-722):
+ * reprojection of the previous result into the current image, then the
+ * ray march of the 1/16 pixels selected by time->frame_count_mod16, then
+ * the ping-pong swap.  Enqueued on `stream` (hipStream_t, NULL = the
+ * context's stream); uniforms are captured by value at enqueue time. */
+int mtc_frame(mtc_ctx* ctx, const mtc_camera* cur, const mtc_camera* old,
+              const mtc_time* time, const mtc_sun* sun,
+              const mtc_cloud_params* params, void* stream);
+/* The ray march alone over every pixel of the frame (no reprojection, no
+ * swap); result lands in the current image. */
+int mtc_march_full(mtc_ctx* ctx, const mtc_camera* cur, const mtc_time* time,
+                   const mtc_sun* sun, const mtc_cloud_params* params, void* stream);
+
+/* Wait for all work of the context. */
+int mtc_synchronize(mtc_ctx* ctx);
+/* Copy the latest result (RGBA32F, W*H*4 floats) and the god-ray mask
+ * (RGBA32F EncodeFloatRGBA, W*H*4 floats).  Either pointer may be NULL. */
+int mtc_read_output(mtc_ctx* ctx, float* rgba, float* mask, size_t n_floats);
+/* Device pointers of the latest result and mask (valid until the next
+ * mtc_frame / mtc_destroy). */
+int mtc_output_device_ptrs(mtc_ctx* ctx, void** rgba, void** mask);
+/* Previous-frame image used by the next reprojection. */
+int mtc_set_history(mtc_ctx* ctx, const float* rgba, size_t n_floats);
+int mtc_reset_history(mtc_ctx* ctx);
+
+/* Counters accumulated since the last reset (needs MTC_FLAG_COUNTERS). */
+int mtc_get_counters(mtc_ctx* ctx, mtc_counters* out);
+int mtc_reset_counters(mtc_ctx* ctx);
+
+/* Reprojection source-texel map for testing: for every pixel and tap the
+ * linear index y*W+x of the previous-frame texel the tap reads
+ * (reprojection.comp:233-240).  idx holds W*H*motion_blur_samples int32. */
+int mtc_reprojection_indices(mtc_ctx* ctx, const mtc_camera* cur, const mtc_camera* old,
+                             const mtc_time* time, const mtc_cloud_params* params,
+                             int32_t* idx, size_t n_idx);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* METEOROS_CLOUD_H */

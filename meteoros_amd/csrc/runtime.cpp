@@ -1,0 +1,389 @@
+// runtime.cpp -- the C ABI of the cloud compute stage (include/meteoros_cloud.h).
+//
+// Replaces the compute half of Renderer (Renderer.cpp:653-722, :1034-1086,
+// :1428-1447) and the texture side of Sky (Sky.cpp:25-58):
+//   * the context owns every device buffer: the four noise resources, the
+//     two RGBA32F result images (ping-pong) and the RGBA32F god-ray mask;
+//   * a frame enqueues reprojection(prev -> cur) then the march of the
+//     selected pixels into cur, in stream order (the reference recorded
+//     both dispatches with no barrier between them, a write-after-write
+//     race; here the order is explicit), then swaps the images;
+//   * uniforms are taken by value at enqueue time (kernel arguments), so
+//     the caller may overwrite its structs as soon as the call returns.
+// Errors are status codes plus a per-thread message; nothing throws across
+// the boundary.
+#include <hip/hip_runtime.h>
+
+#include <cstdio>
+#include <cstring>
+#include <new>
+#include <string>
+
+#include "meteoros_cloud.h"
+#include "runtime_internal.h"
+
+hipError_t launch_march_plain(const FrameConsts& fc, const VolumeViews& vv, const OutputViews& ov, int full,
+                              hipStream_t stream);
+hipError_t launch_reproject(const FrameConsts& fc, const OutputViews& ov, bool indices, hipStream_t stream);
+hipError_t launch_noise(int kind, void* out, int N, int fmt, uint32_t seed, hipStream_t stream);
+
+namespace {
+thread_local std::string g_last_error = "no error";
+
+bool is_pow2(int v) { return v > 0 && (v & (v - 1)) == 0; }
+int log2i(int v) {
+  int s = 0;
+  while ((1 << s) < v) ++s;
+  return s;
+}
+}  // namespace
+
+int mtc_fail(int code, const std::string& msg) {
+  g_last_error = msg;
+  return code;
+}
+
+#define HIP_TRY(expr, what)                                                                            \
+  do {                                                                                                 \
+    hipError_t _e = (expr);                                                                            \
+    if (_e != hipSuccess) {                                                                            \
+      int _code = (_e == hipErrorOutOfMemory) ? MTC_ERR_OUT_OF_MEMORY : MTC_ERR_HIP;                   \
+      return mtc_fail(_code, std::string(what) + ": " + hipGetErrorName(_e) + " (" + hipGetErrorString(_e) + ")"); \
+    }                                                                                                  \
+  } while (0)
+
+struct mtc_ctx {
+  mtc_desc desc;
+  hipStream_t stream = nullptr;
+  void* lf = nullptr;
+  void* hf = nullptr;
+  uint32_t* curl = nullptr;
+  uint32_t* weather = nullptr;
+  float4* img[2] = {nullptr, nullptr};
+  float4* mask = nullptr;
+  unsigned long long* counters = nullptr;
+  int latest = 0;
+  bool volumes_ready = false;
+  size_t lf_bytes = 0, hf_bytes = 0, curl_bytes = 0, weather_bytes = 0;
+};
+
+namespace {
+
+size_t texel_bytes(int fmt) { return fmt == MTC_FMT_FP16 ? 8 : 4; }
+
+void release(mtc_ctx* c) {
+  if (!c) return;
+  (void)hipSetDevice(c->desc.device);
+  if (c->stream) (void)hipStreamSynchronize(c->stream);
+  (void)hipFree(c->lf);
+  (void)hipFree(c->hf);
+  (void)hipFree(c->curl);
+  (void)hipFree(c->weather);
+  (void)hipFree(c->img[0]);
+  (void)hipFree(c->img[1]);
+  (void)hipFree(c->mask);
+  (void)hipFree(c->counters);
+  if (c->stream) (void)hipStreamDestroy(c->stream);
+  delete c;
+}
+
+VolumeViews volume_views(const mtc_ctx* c) {
+  VolumeViews v;
+  v.lf = c->lf;
+  v.hf = c->hf;
+  v.curl = c->curl;
+  v.weather = c->weather;
+  v.lf_dim = c->desc.volumes.lf_dim;
+  v.hf_dim = c->desc.volumes.hf_dim;
+  v.curl_dim = c->desc.volumes.curl_dim;
+  v.weather_dim = c->desc.volumes.weather_dim;
+  v.lf_shift = log2i(v.lf_dim);
+  v.hf_shift = log2i(v.hf_dim);
+  v.curl_shift = log2i(v.curl_dim);
+  v.weather_shift = log2i(v.weather_dim);
+  v.format = c->desc.volumes.format;
+  return v;
+}
+
+hipStream_t pick_stream(mtc_ctx* c, void* s) { return s ? (hipStream_t)s : c->stream; }
+
+int check_ctx(mtc_ctx* c, const char* fn) {
+  if (!c) return mtc_fail(MTC_ERR_INVALID_ARG, std::string(fn) + ": null context");
+  hipError_t e = hipSetDevice(c->desc.device);
+  if (e != hipSuccess) return mtc_fail(MTC_ERR_HIP, std::string(fn) + ": hipSetDevice failed");
+  return MTC_OK;
+}
+
+int launch_march(mtc_ctx* c, const FrameConsts& fc, const OutputViews& ov, int full, hipStream_t s) {
+  VolumeViews vv = volume_views(c);
+  hipError_t e;
+  switch (c->desc.kernel) {
+    case MTC_KERNEL_PLAIN: e = launch_march_plain(fc, vv, ov, full, s); break;
+    default: return mtc_fail(MTC_ERR_UNSUPPORTED, "march: kernel variant not available in this build");
+  }
+  HIP_TRY(e, "march kernel launch");
+  return MTC_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+const char* mtc_last_error(void) { return g_last_error.c_str(); }
+
+const char* mtc_version(void) { return "meteoros_cloud 0.1 (gfx950)"; }
+
+int mtc_device_count(int32_t* count) {
+  if (!count) return mtc_fail(MTC_ERR_INVALID_ARG, "mtc_device_count: null pointer");
+  int n = 0;
+  if (hipGetDeviceCount(&n) != hipSuccess) n = 0;
+  *count = n;
+  return MTC_OK;
+}
+
+int mtc_create(const mtc_desc* d, mtc_ctx** out) {
+  if (!d || !out) return mtc_fail(MTC_ERR_INVALID_ARG, "mtc_create: null pointer");
+  *out = nullptr;
+  if (d->width <= 0 || d->height <= 0 || d->width > 16384 || d->height > 16384)
+    return mtc_fail(MTC_ERR_INVALID_ARG, "mtc_create: width/height must be in 1..16384");
+  const mtc_volume_desc& v = d->volumes;
+  if (!is_pow2(v.lf_dim) || !is_pow2(v.hf_dim) || !is_pow2(v.curl_dim) || !is_pow2(v.weather_dim))
+    return mtc_fail(MTC_ERR_INVALID_ARG, "mtc_create: volume and map dimensions must be powers of two");
+  if (v.lf_dim > 1024 || v.hf_dim > 1024 || v.curl_dim > 8192 || v.weather_dim > 8192)
+    return mtc_fail(MTC_ERR_INVALID_ARG, "mtc_create: volume dimension too large");
+  if (v.format != MTC_FMT_UNORM8 && v.format != MTC_FMT_FP16)
+    return mtc_fail(MTC_ERR_INVALID_ARG, "mtc_create: unknown volume format");
+  if (d->kernel < MTC_KERNEL_PLAIN || d->kernel > MTC_KERNEL_LEARNED)
+    return mtc_fail(MTC_ERR_INVALID_ARG, "mtc_create: unknown kernel variant");
+  if (d->kernel == MTC_KERNEL_LEARNED) return mtc_fail(MTC_ERR_UNSUPPORTED, "mtc_create: learned-light kernel not built");
+  if (d->flags & ~(MTC_FLAG_COUNTERS | MTC_FLAG_FULL_MARCH)) return mtc_fail(MTC_ERR_INVALID_ARG, "mtc_create: unknown flags");
+  int ndev = 0;
+  if (hipGetDeviceCount(&ndev) != hipSuccess || ndev <= 0)
+    return mtc_fail(MTC_ERR_HIP, "mtc_create: no HIP device available");
+  if (d->device < 0 || d->device >= ndev) return mtc_fail(MTC_ERR_INVALID_ARG, "mtc_create: device ordinal out of range");
+  HIP_TRY(hipSetDevice(d->device), "mtc_create: hipSetDevice");
+
+  mtc_ctx* c = new (std::nothrow) mtc_ctx;
+  if (!c) return mtc_fail(MTC_ERR_OUT_OF_MEMORY, "mtc_create: host allocation failed");
+  c->desc = *d;
+  c->lf_bytes = (size_t)v.lf_dim * v.lf_dim * v.lf_dim * texel_bytes(v.format);
+  c->hf_bytes = (size_t)v.hf_dim * v.hf_dim * v.hf_dim * texel_bytes(v.format);
+  c->curl_bytes = (size_t)v.curl_dim * v.curl_dim * 4;
+  c->weather_bytes = (size_t)v.weather_dim * v.weather_dim * 4;
+  const size_t px = (size_t)d->width * d->height;
+  hipError_t e = hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking);
+  if (e == hipSuccess) e = hipMalloc(&c->lf, c->lf_bytes);
+  if (e == hipSuccess) e = hipMalloc(&c->hf, c->hf_bytes);
+  if (e == hipSuccess) e = hipMalloc((void**)&c->curl, c->curl_bytes);
+  if (e == hipSuccess) e = hipMalloc((void**)&c->weather, c->weather_bytes);
+  if (e == hipSuccess) e = hipMalloc((void**)&c->img[0], px * sizeof(float4));
+  if (e == hipSuccess) e = hipMalloc((void**)&c->img[1], px * sizeof(float4));
+  if (e == hipSuccess) e = hipMalloc((void**)&c->mask, px * sizeof(float4));
+  if (e == hipSuccess) e = hipMalloc((void**)&c->counters, 16 * sizeof(unsigned long long));
+  if (e == hipSuccess) e = hipMemsetAsync(c->img[0], 0, px * sizeof(float4), c->stream);
+  if (e == hipSuccess) e = hipMemsetAsync(c->img[1], 0, px * sizeof(float4), c->stream);
+  if (e == hipSuccess) e = hipMemsetAsync(c->mask, 0, px * sizeof(float4), c->stream);
+  if (e == hipSuccess) e = hipMemsetAsync(c->counters, 0, 16 * sizeof(unsigned long long), c->stream);
+  if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
+  if (e != hipSuccess) {
+    int code = e == hipErrorOutOfMemory ? MTC_ERR_OUT_OF_MEMORY : MTC_ERR_HIP;
+    std::string msg = std::string("mtc_create: ") + hipGetErrorName(e);
+    release(c);
+    return mtc_fail(code, msg);
+  }
+  *out = c;
+  return MTC_OK;
+}
+
+int mtc_destroy(mtc_ctx* c) {
+  if (!c) return MTC_OK;
+  release(c);
+  return MTC_OK;
+}
+
+int mtc_set_volumes(mtc_ctx* c, const void* lf, const void* hf, const void* curl, const void* weather) {
+  int rc = check_ctx(c, "mtc_set_volumes");
+  if (rc) return rc;
+  if (!lf || !hf || !curl || !weather) return mtc_fail(MTC_ERR_INVALID_ARG, "mtc_set_volumes: null volume pointer");
+  HIP_TRY(hipMemcpyAsync(c->lf, lf, c->lf_bytes, hipMemcpyHostToDevice, c->stream), "mtc_set_volumes: lf");
+  HIP_TRY(hipMemcpyAsync(c->hf, hf, c->hf_bytes, hipMemcpyHostToDevice, c->stream), "mtc_set_volumes: hf");
+  HIP_TRY(hipMemcpyAsync(c->curl, curl, c->curl_bytes, hipMemcpyHostToDevice, c->stream), "mtc_set_volumes: curl");
+  HIP_TRY(hipMemcpyAsync(c->weather, weather, c->weather_bytes, hipMemcpyHostToDevice, c->stream),
+          "mtc_set_volumes: weather");
+  HIP_TRY(hipStreamSynchronize(c->stream), "mtc_set_volumes: sync");
+  c->volumes_ready = true;
+  return MTC_OK;
+}
+
+int mtc_generate_volumes(mtc_ctx* c, const mtc_noise_desc* nd) {
+  int rc = check_ctx(c, "mtc_generate_volumes");
+  if (rc) return rc;
+  if (!nd) return mtc_fail(MTC_ERR_INVALID_ARG, "mtc_generate_volumes: null noise desc");
+  const mtc_volume_desc& v = c->desc.volumes;
+  HIP_TRY(launch_noise(0, c->lf, v.lf_dim, v.format, nd->seed_volumes, c->stream), "noise lf");
+  HIP_TRY(launch_noise(1, c->hf, v.hf_dim, v.format, nd->seed_volumes, c->stream), "noise hf");
+  HIP_TRY(launch_noise(2, c->curl, v.curl_dim, MTC_FMT_UNORM8, nd->seed_volumes, c->stream), "noise curl");
+  HIP_TRY(launch_noise(3, c->weather, v.weather_dim, MTC_FMT_UNORM8, nd->seed_weather, c->stream), "noise weather");
+  HIP_TRY(hipStreamSynchronize(c->stream), "mtc_generate_volumes: sync");
+  c->volumes_ready = true;
+  return MTC_OK;
+}
+
+int mtc_get_volumes(mtc_ctx* c, void* lf, void* hf, void* curl, void* weather) {
+  int rc = check_ctx(c, "mtc_get_volumes");
+  if (rc) return rc;
+  if (!c->volumes_ready) return mtc_fail(MTC_ERR_STATE, "mtc_get_volumes: no volumes resident");
+  HIP_TRY(hipStreamSynchronize(c->stream), "mtc_get_volumes: sync");
+  if (lf) HIP_TRY(hipMemcpy(lf, c->lf, c->lf_bytes, hipMemcpyDeviceToHost), "mtc_get_volumes: lf");
+  if (hf) HIP_TRY(hipMemcpy(hf, c->hf, c->hf_bytes, hipMemcpyDeviceToHost), "mtc_get_volumes: hf");
+  if (curl) HIP_TRY(hipMemcpy(curl, c->curl, c->curl_bytes, hipMemcpyDeviceToHost), "mtc_get_volumes: curl");
+  if (weather) HIP_TRY(hipMemcpy(weather, c->weather, c->weather_bytes, hipMemcpyDeviceToHost), "mtc_get_volumes: weather");
+  return MTC_OK;
+}
+
+int mtc_frame(mtc_ctx* c, const mtc_camera* cur, const mtc_camera* old, const mtc_time* time, const mtc_sun* sun,
+              const mtc_cloud_params* params, void* stream) {
+  int rc = check_ctx(c, "mtc_frame");
+  if (rc) return rc;
+  if (!c->volumes_ready) return mtc_fail(MTC_ERR_STATE, "mtc_frame: volumes not set or generated");
+  if (!old) return mtc_fail(MTC_ERR_INVALID_ARG, "mtc_frame: null old camera");
+  FrameConsts fc;
+  rc = build_frame_consts(cur, old, time, sun, params, c->desc.width, c->desc.height, &fc);
+  if (rc) return rc;
+  hipStream_t s = pick_stream(c, stream);
+  const int dst = 1 - c->latest;
+  OutputViews ov;
+  ov.color = c->img[dst];
+  ov.prev = c->img[c->latest];
+  ov.mask = c->mask;
+  ov.counters = (c->desc.flags & MTC_FLAG_COUNTERS) ? c->counters : nullptr;
+  ov.reproj_idx = nullptr;
+  const int full = (c->desc.flags & MTC_FLAG_FULL_MARCH) ? 1 : 0;
+  if (!full) HIP_TRY(launch_reproject(fc, ov, false, s), "reprojection kernel launch");
+  rc = launch_march(c, fc, ov, full, s);
+  if (rc) return rc;
+  c->latest = dst;
+  return MTC_OK;
+}
+
+int mtc_march_full(mtc_ctx* c, const mtc_camera* cur, const mtc_time* time, const mtc_sun* sun,
+                   const mtc_cloud_params* params, void* stream) {
+  int rc = check_ctx(c, "mtc_march_full");
+  if (rc) return rc;
+  if (!c->volumes_ready) return mtc_fail(MTC_ERR_STATE, "mtc_march_full: volumes not set or generated");
+  FrameConsts fc;
+  rc = build_frame_consts(cur, nullptr, time, sun, params, c->desc.width, c->desc.height, &fc);
+  if (rc) return rc;
+  OutputViews ov;
+  ov.color = c->img[c->latest];
+  ov.prev = nullptr;
+  ov.mask = c->mask;
+  ov.counters = (c->desc.flags & MTC_FLAG_COUNTERS) ? c->counters : nullptr;
+  ov.reproj_idx = nullptr;
+  return launch_march(c, fc, ov, 1, pick_stream(c, stream));
+}
+
+int mtc_synchronize(mtc_ctx* c) {
+  int rc = check_ctx(c, "mtc_synchronize");
+  if (rc) return rc;
+  HIP_TRY(hipStreamSynchronize(c->stream), "mtc_synchronize");
+  HIP_TRY(hipDeviceSynchronize(), "mtc_synchronize: device");
+  return MTC_OK;
+}
+
+int mtc_read_output(mtc_ctx* c, float* rgba, float* mask, size_t n_floats) {
+  int rc = check_ctx(c, "mtc_read_output");
+  if (rc) return rc;
+  const size_t need = (size_t)c->desc.width * c->desc.height * 4;
+  if (n_floats < need) return mtc_fail(MTC_ERR_INVALID_ARG, "mtc_read_output: buffer smaller than W*H*4 floats");
+  HIP_TRY(hipDeviceSynchronize(), "mtc_read_output: sync");
+  if (rgba) HIP_TRY(hipMemcpy(rgba, c->img[c->latest], need * sizeof(float), hipMemcpyDeviceToHost), "mtc_read_output: rgba");
+  if (mask) HIP_TRY(hipMemcpy(mask, c->mask, need * sizeof(float), hipMemcpyDeviceToHost), "mtc_read_output: mask");
+  return MTC_OK;
+}
+
+int mtc_output_device_ptrs(mtc_ctx* c, void** rgba, void** mask) {
+  if (!c) return mtc_fail(MTC_ERR_INVALID_ARG, "mtc_output_device_ptrs: null context");
+  if (rgba) *rgba = c->img[c->latest];
+  if (mask) *mask = c->mask;
+  return MTC_OK;
+}
+
+int mtc_set_history(mtc_ctx* c, const float* rgba, size_t n_floats) {
+  int rc = check_ctx(c, "mtc_set_history");
+  if (rc) return rc;
+  const size_t need = (size_t)c->desc.width * c->desc.height * 4;
+  if (!rgba || n_floats != need) return mtc_fail(MTC_ERR_INVALID_ARG, "mtc_set_history: need exactly W*H*4 floats");
+  HIP_TRY(hipDeviceSynchronize(), "mtc_set_history: sync");
+  HIP_TRY(hipMemcpy(c->img[c->latest], rgba, need * sizeof(float), hipMemcpyHostToDevice), "mtc_set_history");
+  return MTC_OK;
+}
+
+int mtc_reset_history(mtc_ctx* c) {
+  int rc = check_ctx(c, "mtc_reset_history");
+  if (rc) return rc;
+  const size_t px = (size_t)c->desc.width * c->desc.height;
+  HIP_TRY(hipMemsetAsync(c->img[0], 0, px * sizeof(float4), c->stream), "mtc_reset_history");
+  HIP_TRY(hipMemsetAsync(c->img[1], 0, px * sizeof(float4), c->stream), "mtc_reset_history");
+  HIP_TRY(hipMemsetAsync(c->mask, 0, px * sizeof(float4), c->stream), "mtc_reset_history");
+  HIP_TRY(hipStreamSynchronize(c->stream), "mtc_reset_history: sync");
+  c->latest = 0;
+  return MTC_OK;
+}
+
+int mtc_get_counters(mtc_ctx* c, mtc_counters* out) {
+  int rc = check_ctx(c, "mtc_get_counters");
+  if (rc) return rc;
+  if (!out) return mtc_fail(MTC_ERR_INVALID_ARG, "mtc_get_counters: null pointer");
+  if (!(c->desc.flags & MTC_FLAG_COUNTERS)) return mtc_fail(MTC_ERR_STATE, "mtc_get_counters: context built without MTC_FLAG_COUNTERS");
+  unsigned long long h[16];
+  HIP_TRY(hipDeviceSynchronize(), "mtc_get_counters: sync");
+  HIP_TRY(hipMemcpy(h, c->counters, sizeof(h), hipMemcpyDeviceToHost), "mtc_get_counters");
+  out->rays_marched = h[0];
+  out->rays_ocean = h[1];
+  out->rays_sky = h[2];
+  out->primary_steps = h[3];
+  out->cloud_steps = h[4];
+  out->cone_samples = h[5];
+  out->cone_erosions = h[6];
+  out->early_exits = h[7];
+  out->mlp_queries = h[8];
+  out->mlp_fallbacks = h[9];
+  return MTC_OK;
+}
+
+int mtc_reset_counters(mtc_ctx* c) {
+  int rc = check_ctx(c, "mtc_reset_counters");
+  if (rc) return rc;
+  HIP_TRY(hipMemsetAsync(c->counters, 0, 16 * sizeof(unsigned long long), c->stream), "mtc_reset_counters");
+  HIP_TRY(hipStreamSynchronize(c->stream), "mtc_reset_counters: sync");
+  return MTC_OK;
+}
+
+int mtc_reprojection_indices(mtc_ctx* c, const mtc_camera* cur, const mtc_camera* old, const mtc_time* time,
+                             const mtc_cloud_params* params, int32_t* idx, size_t n_idx) {
+  int rc = check_ctx(c, "mtc_reprojection_indices");
+  if (rc) return rc;
+  if (!idx || !old || !params) return mtc_fail(MTC_ERR_INVALID_ARG, "mtc_reprojection_indices: null pointer");
+  const size_t need = (size_t)c->desc.width * c->desc.height * (size_t)params->motion_blur_samples;
+  if (n_idx != need) return mtc_fail(MTC_ERR_INVALID_ARG, "mtc_reprojection_indices: need W*H*motion_blur_samples entries");
+  mtc_sun sun;
+  std::memset(&sun, 0, sizeof(sun));
+  sun.sun_location[1] = 1.0f;
+  FrameConsts fc;
+  rc = build_frame_consts(cur, old, time, &sun, params, c->desc.width, c->desc.height, &fc);
+  if (rc) return rc;
+  int32_t* d = nullptr;
+  HIP_TRY(hipMalloc((void**)&d, need * sizeof(int32_t)), "mtc_reprojection_indices: alloc");
+  OutputViews ov;
+  std::memset(&ov, 0, sizeof(ov));
+  ov.reproj_idx = d;
+  hipError_t e = launch_reproject(fc, ov, true, c->stream);
+  if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
+  if (e == hipSuccess) e = hipMemcpy(idx, d, need * sizeof(int32_t), hipMemcpyDeviceToHost);
+  (void)hipFree(d);
+  HIP_TRY(e, "mtc_reprojection_indices");
+  return MTC_OK;
+}
+
+}  // extern "C"

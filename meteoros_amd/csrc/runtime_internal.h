@@ -1,0 +1,73 @@
+// runtime_internal.h -- shared declarations of the host runtime.
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+#include <string>
+
+#include "meteoros_cloud.h"
+
+// Record `msg` as this thread's last error and return `code`.
+int mtc_fail(int code, const std::string& msg);
+
+// Per-frame constants computed once on the host (in FP32 with the same
+// formulas the per-ray code would use) and passed to the kernels by value.
+// Everything here is the same for every ray of a frame, because every ray
+// starts at the eye (cloudRayMarch.comp:222, :707).
+struct FrameConsts {
+  // camera (cloudRayMarch.comp:15-21, :199-207)
+  float view[16];
+  float old_view[16];
+  float eye[3];  // ray origin = -camera.eye.xyz
+  float tfx, tfy;
+  float jit_march[2];   // getJitterOffset(frame, dim) numerators, march table (:114-132)
+  float jit_reproj[2];  // same, reprojection table (reprojection.comp:72-90)
+  int32_t frame;        // frameCountMod16
+  int32_t width, height;
+  // per-step direction jitter (:631-633): (jx, (jx+jy)*jitter_factor, jy)
+  // for _index = 0..15, jx, jy = getJitterOffset(_index, ivec2(75))
+  float step_jitter[16][3];
+  // geometry
+  float earth_r, r_in, r_out, thickness, inner_off;
+  double earth_r_d, r_in_d, r_out_d;
+  // march controls
+  int32_t step_mode, steps_min, steps_max, light_samples, coverage_mode, blur_samples;
+  float base_density_factor, sample_divisor, erosion_scale, accum_scale, cone_erosion_scale, brightness;
+  float hg_g, hg_si, hg_g2;  // hg_g2 = 0.99 - silver_spread
+  float wind_dir[3], top_offset, wind_skew;
+  float wind_offset[3];  // ((wind_dir + (0, bias, 0)) * cloud_speed) * total_time
+  float coverage_const, weather_scale;
+  float bg_min, fade_start, fade_end;
+  // lighting (:585-624): lightDir = normalize(SUN_LOCATION - eye), cone kernel
+  float light_dir[3];
+  float kern[6][3];
+  // Preetham background (:401-467): per-frame parts
+  float sky_sun_dir[3];  // normalize(BACKGROUND_SKY_SUN_LOCATION - eye)
+  float sunE;            // sunIntensity * calcSunIntensity()
+  float beta_r[3], beta_m[3], beta_sum[3];
+  float ydot_clamped;    // clamp((1 - sunDir.y)^5, 0, 1)
+  float sun_disk_lo, sun_disk_hi;
+};
+
+// Fills FrameConsts from the boundary structs (runtime_frame.cpp).
+int build_frame_consts(const mtc_camera* cur, const mtc_camera* old, const mtc_time* time, const mtc_sun* sun,
+                       const mtc_cloud_params* p, int width, int height, FrameConsts* fc);
+
+// Device views of the resident resources.
+struct VolumeViews {
+  const void* lf;        // uint32 RGBA8 texels or 4 x half texels
+  const void* hf;
+  const uint32_t* curl;  // RGBA8
+  const uint32_t* weather;
+  int32_t lf_dim, hf_dim, curl_dim, weather_dim;
+  int32_t lf_shift, hf_shift, curl_shift, weather_shift;  // log2 of the dims
+  int32_t format;
+};
+
+struct OutputViews {
+  float4* color;      // current result image, W*H RGBA32F
+  const float4* prev; // previous result image
+  float4* mask;       // god-ray mask, W*H RGBA32F
+  unsigned long long* counters;  // 10 x u64 or nullptr
+  int32_t* reproj_idx;           // optional tap-index map
+};

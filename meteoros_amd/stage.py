@@ -1,0 +1,202 @@
+"""Host-side mirror of the reference's cloud-pass interface.
+
+The reference drives its compute stage from C++ objects (camera.{h,cpp},
+Scene.{h,cpp}, Sky.{h,cpp}, Renderer.{h,cpp}); the same roles exist here as
+thin Python classes over the C ABI of libmeteoros_cloud.so:
+
+  Camera      camera.h:20-68      lookAt/perspective uniforms, RotateAboutUp ...
+  Time        Scene.h:12-21       Halton table, (dt, total), frameCount mod 16
+  CloudStage  Renderer.cpp:653-722 + Sky.cpp:25-58
+              volumes, frame() = reprojection then march, ping-pong history
+
+Every call goes to the native library; nothing here computes the stage.
+"""
+from __future__ import annotations
+
+import ctypes as C
+from dataclasses import dataclass
+
+import numpy as np
+
+from . import _abi as A
+from ._abi import check, lib
+
+
+class Camera:
+    """camera.h:20-68: eye/ref/up basis, UpdateBuffer() -> CameraUBO."""
+
+    def __init__(self, eye, ref, width: int, height: int, fovy_deg: float = 45.0,
+                 near_clip: float = 0.1, far_clip: float = 1000.0):
+        self.state = A.MtcCameraState()
+        e = (C.c_float * 3)(*eye)
+        r = (C.c_float * 3)(*ref)
+        check(lib().mtc_camera_init(C.byref(self.state), e, r, int(width), int(height), float(fovy_deg),
+                                    float(near_clip), float(far_clip)), "mtc_camera_init")
+
+    def rotate_about_up(self, deg: float) -> None:
+        check(lib().mtc_camera_rotate_about_up(C.byref(self.state), float(deg)), "mtc_camera_rotate_about_up")
+
+    def rotate_about_right(self, deg: float) -> None:
+        check(lib().mtc_camera_rotate_about_right(C.byref(self.state), float(deg)), "mtc_camera_rotate_about_right")
+
+    def translate(self, look: float = 0.0, right: float = 0.0, up: float = 0.0) -> None:
+        check(lib().mtc_camera_translate(C.byref(self.state), float(look), float(right), float(up)),
+              "mtc_camera_translate")
+
+    def ubo(self) -> A.MtcCamera:
+        out = A.MtcCamera()
+        check(lib().mtc_camera_ubo(C.byref(self.state), C.byref(out)), "mtc_camera_ubo")
+        return out
+
+
+class Time:
+    """Scene.h:12-21 / Scene.cpp:65-119 with the wall clock replaced by dt."""
+
+    def __init__(self):
+        self.ubo = A.MtcTime()
+        check(lib().mtc_time_init(C.byref(self.ubo)), "mtc_time_init")
+
+    def advance(self, dt: float) -> None:
+        check(lib().mtc_time_advance(C.byref(self.ubo), float(dt)), "mtc_time_advance")
+
+    @property
+    def frame(self) -> int:
+        return int(self.ubo.frame_count_mod16)
+
+
+def halton(index: int, base: int) -> float:
+    """Scene::HaltonSequenceAt (Scene.cpp:125-138)."""
+    return float(lib().mtc_halton(int(index), int(base)))
+
+
+def reference_params():
+    """The shipped #define values of cloudRayMarch.comp / reprojection.comp."""
+    p, s = A.MtcCloudParams(), A.MtcSun()
+    check(lib().mtc_params_reference(C.byref(p), C.byref(s)), "mtc_params_reference")
+    return p, s
+
+
+def preset_params(preset: int):
+    """Workload presets 1..5 (shell R+1500..R+4000 m, weather coverage,
+    sun at 30 deg, 10 m/s wind; 64/128/192/64/128 primary steps)."""
+    p, s = A.MtcCloudParams(), A.MtcSun()
+    check(lib().mtc_params_preset(int(preset), C.byref(p), C.byref(s)), "mtc_params_preset")
+    return p, s
+
+
+@dataclass
+class VolumeSpec:
+    lf_dim: int = 128
+    hf_dim: int = 32
+    curl_dim: int = 128
+    weather_dim: int = 512
+    fmt: int = A.MTC_FMT_UNORM8
+
+    def desc(self) -> A.MtcVolumeDesc:
+        return A.MtcVolumeDesc(self.lf_dim, self.hf_dim, self.curl_dim, self.weather_dim, self.fmt)
+
+
+def device_count() -> int:
+    n = C.c_int32(0)
+    check(lib().mtc_device_count(C.byref(n)), "mtc_device_count")
+    return int(n.value)
+
+
+class CloudStage:
+    """The compute half of Renderer: resources + Frame() for one context."""
+
+    def __init__(self, width: int, height: int, volumes: VolumeSpec | None = None,
+                 kernel: int = A.MTC_KERNEL_PLAIN, flags: int = 0, device: int = 0):
+        self.width, self.height = int(width), int(height)
+        self.volumes = volumes or VolumeSpec()
+        d = A.MtcDesc()
+        d.width, d.height = self.width, self.height
+        d.volumes = self.volumes.desc()
+        d.kernel, d.flags, d.device = int(kernel), int(flags), int(device)
+        self._ctx = C.c_void_p()
+        check(lib().mtc_create(C.byref(d), C.byref(self._ctx)), "mtc_create")
+
+    def close(self) -> None:
+        if self._ctx:
+            lib().mtc_destroy(self._ctx)
+            self._ctx = C.c_void_p()
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    @property
+    def handle(self) -> C.c_void_p:
+        return self._ctx
+
+    # -- resources (Sky::CreateCloudResources) --------------------------------
+    def set_volumes(self, lf: np.ndarray, hf: np.ndarray, curl: np.ndarray, weather: np.ndarray) -> None:
+        arrs = [np.ascontiguousarray(a) for a in (lf, hf, curl, weather)]
+        check(lib().mtc_set_volumes(self._ctx, *[a.ctypes.data for a in arrs]), "mtc_set_volumes")
+
+    def generate_volumes(self, seed_volumes: int = 1, seed_weather: int = 2) -> None:
+        nd = A.MtcNoiseDesc(seed_volumes, seed_weather)
+        check(lib().mtc_generate_volumes(self._ctx, C.byref(nd)), "mtc_generate_volumes")
+
+    def get_volumes(self):
+        v = self.volumes
+        tb = 8 if v.fmt == A.MTC_FMT_FP16 else 4
+        dt = np.uint16 if v.fmt == A.MTC_FMT_FP16 else np.uint8
+        lf = np.empty(v.lf_dim ** 3 * tb // dt().itemsize, dtype=dt)
+        hf = np.empty(v.hf_dim ** 3 * tb // dt().itemsize, dtype=dt)
+        curl = np.empty(v.curl_dim ** 2 * 4, dtype=np.uint8)
+        weather = np.empty(v.weather_dim ** 2 * 4, dtype=np.uint8)
+        check(lib().mtc_get_volumes(self._ctx, lf.ctypes.data, hf.ctypes.data, curl.ctypes.data,
+                                    weather.ctypes.data), "mtc_get_volumes")
+        return lf, hf, curl, weather
+
+    # -- frames (Renderer::Frame) ---------------------------------------------
+    def frame(self, cam: A.MtcCamera, old: A.MtcCamera, time: A.MtcTime, sun: A.MtcSun,
+              params: A.MtcCloudParams, stream: int | None = None) -> None:
+        check(lib().mtc_frame(self._ctx, C.byref(cam), C.byref(old), C.byref(time), C.byref(sun),
+                              C.byref(params), stream), "mtc_frame")
+
+    def march_full(self, cam: A.MtcCamera, time: A.MtcTime, sun: A.MtcSun, params: A.MtcCloudParams,
+                   stream: int | None = None) -> None:
+        check(lib().mtc_march_full(self._ctx, C.byref(cam), C.byref(time), C.byref(sun), C.byref(params), stream),
+              "mtc_march_full")
+
+    def synchronize(self) -> None:
+        check(lib().mtc_synchronize(self._ctx), "mtc_synchronize")
+
+    def read_output(self):
+        n = self.width * self.height * 4
+        rgba = np.empty(n, dtype=np.float32)
+        mask = np.empty(n, dtype=np.float32)
+        check(lib().mtc_read_output(self._ctx, rgba.ctypes.data, mask.ctypes.data, n), "mtc_read_output")
+        return rgba.reshape(self.height, self.width, 4), mask.reshape(self.height, self.width, 4)
+
+    def set_history(self, rgba: np.ndarray) -> None:
+        a = np.ascontiguousarray(rgba, dtype=np.float32).reshape(-1)
+        check(lib().mtc_set_history(self._ctx, a.ctypes.data, a.size), "mtc_set_history")
+
+    def reset_history(self) -> None:
+        check(lib().mtc_reset_history(self._ctx), "mtc_reset_history")
+
+    def counters(self) -> dict:
+        c = A.MtcCounters()
+        check(lib().mtc_get_counters(self._ctx, C.byref(c)), "mtc_get_counters")
+        return {name: int(getattr(c, name)) for name, _ in A.MtcCounters._fields_}
+
+    def reset_counters(self) -> None:
+        check(lib().mtc_reset_counters(self._ctx), "mtc_reset_counters")
+
+    def reprojection_indices(self, cam, old, time, params) -> np.ndarray:
+        n = self.width * self.height * int(params.motion_blur_samples)
+        idx = np.empty(n, dtype=np.int32)
+        check(lib().mtc_reprojection_indices(self._ctx, C.byref(cam), C.byref(old), C.byref(time), C.byref(params),
+                                             idx.ctypes.data, n), "mtc_reprojection_indices")
+        return idx.reshape(self.height, self.width, int(params.motion_blur_samples))

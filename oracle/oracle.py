@@ -1,0 +1,218 @@
+"""Python binding of the CPU oracle (oracle/_build/liboracle.so).
+
+TEST INFRASTRUCTURE ONLY.  Imported by tests/, __graft_entry__.smoke() and
+bench.py's cpu_baseline leg, where it is the checker (or the timed CPU
+baseline); the product package never imports it.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+import subprocess
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "_build", "liboracle.so")
+
+# The boundary structs are shared with the product's C header; import the
+# ctypes mirrors from there (they are plain type definitions).
+from meteoros_amd import _abi as A  # noqa: E402
+
+_lib = None
+P = C.POINTER
+
+
+def build() -> None:
+    subprocess.run(["make", "-s", "-C", _HERE], check=True)
+
+
+def lib() -> C.CDLL:
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise ImportError(f"{LIB_PATH} missing; run `make -C oracle`")
+        h = C.CDLL(LIB_PATH)
+        vp = C.c_void_p
+        h.orc_march_pixels.restype = C.c_int
+        h.orc_march_pixels.argtypes = [vp, vp, vp, vp, P(A.MtcVolumeDesc), P(A.MtcCamera), P(A.MtcTime), P(A.MtcSun),
+                                       P(A.MtcCloudParams), C.c_int, C.c_int, vp, C.c_int64, C.c_int, vp, vp, vp, vp,
+                                       C.c_int]
+        h.orc_select_pixels.restype = C.c_int64
+        h.orc_select_pixels.argtypes = [C.c_int, C.c_int, C.c_int, vp]
+        h.orc_frame.restype = C.c_int
+        h.orc_frame.argtypes = [vp, vp, vp, vp, P(A.MtcVolumeDesc), P(A.MtcCamera), P(A.MtcCamera), P(A.MtcTime),
+                                P(A.MtcSun), P(A.MtcCloudParams), C.c_int, C.c_int, C.c_int, C.c_int, vp, vp, vp, vp,
+                                C.c_int]
+        h.orc_reprojection_indices.restype = C.c_int
+        h.orc_reprojection_indices.argtypes = [P(A.MtcCamera), P(A.MtcCamera), P(A.MtcTime), P(A.MtcCloudParams),
+                                               C.c_int, C.c_int, C.c_int, vp, C.c_int]
+        h.orc_encode_float_rgba.restype = None
+        h.orc_encode_float_rgba.argtypes = [C.c_float, P(C.c_float)]
+        h.orc_encode_float_rgba64.restype = None
+        h.orc_encode_float_rgba64.argtypes = [C.c_double, P(C.c_double)]
+        h.orc_hg.restype = C.c_double
+        h.orc_hg.argtypes = [C.c_double, C.c_double, C.c_int]
+        h.orc_remap.restype = C.c_double
+        h.orc_remap.argtypes = [C.c_double] * 5 + [C.c_int]
+        h.orc_smoothstep.restype = C.c_double
+        h.orc_smoothstep.argtypes = [C.c_double] * 3
+        h.orc_ray_sphere.restype = None
+        h.orc_ray_sphere.argtypes = [P(C.c_double), P(C.c_double), P(C.c_double), C.c_double, C.c_int, P(C.c_double)]
+        h.orc_sample3d.restype = None
+        h.orc_sample3d.argtypes = [vp, C.c_int, C.c_int, P(C.c_double), C.c_int, P(C.c_double)]
+        h.orc_sample2d.restype = None
+        h.orc_sample2d.argtypes = [vp, C.c_int, C.c_double, C.c_double, C.c_int, P(C.c_double)]
+        h.orc_atmosphere.restype = None
+        h.orc_atmosphere.argtypes = [P(A.MtcSun), P(A.MtcCloudParams), P(C.c_double), P(C.c_double), C.c_int,
+                                     P(C.c_double)]
+        h.orc_noise_generate.restype = C.c_int
+        h.orc_noise_generate.argtypes = [C.c_int, C.c_int, C.c_int, C.c_uint32, C.c_int, C.c_int, vp, C.c_int]
+        h.orc_to_half.restype = C.c_uint16
+        h.orc_to_half.argtypes = [C.c_float]
+        _lib = h
+    return _lib
+
+
+def _d3(v):
+    return (C.c_double * 3)(*[float(x) for x in v])
+
+
+COUNTER_NAMES = ("rays_marched", "rays_ocean", "rays_sky", "primary_steps", "cloud_steps", "cone_samples",
+                 "cone_erosions", "early_exits")
+
+
+class Volumes:
+    """The four stage resources as numpy arrays (x fastest, 4 channels)."""
+
+    def __init__(self, lf, hf, curl, weather, lf_dim, hf_dim, curl_dim, weather_dim, fmt=A.MTC_FMT_UNORM8):
+        self.lf, self.hf = np.ascontiguousarray(lf), np.ascontiguousarray(hf)
+        self.curl, self.weather = np.ascontiguousarray(curl), np.ascontiguousarray(weather)
+        self.desc = A.MtcVolumeDesc(lf_dim, hf_dim, curl_dim, weather_dim, fmt)
+
+    def ptrs(self):
+        return (self.lf.ctypes.data, self.hf.ctypes.data, self.curl.ctypes.data, self.weather.ctypes.data)
+
+
+def noise(kind: int, n: int, fmt: int = A.MTC_FMT_UNORM8, seed: int = 1, z0: int = 0, nz: int = 0,
+          threads: int = 0) -> np.ndarray:
+    """CPU generator: kind 0 = LF (n^3), 1 = HF (n^3), 2 = curl (n^2), 3 = weather (n^2)."""
+    depth = (n if nz <= 0 else nz) if kind <= 1 else 1
+    count = depth * n * n * 4
+    out = np.empty(count, dtype=np.uint16 if fmt == A.MTC_FMT_FP16 else np.uint8)
+    rc = lib().orc_noise_generate(kind, n, fmt, seed, z0, nz, out.ctypes.data, threads)
+    if rc:
+        raise ValueError("orc_noise_generate rejected its arguments")
+    return out
+
+
+def generate_volumes(lf_dim=128, hf_dim=32, curl_dim=128, weather_dim=512, fmt=A.MTC_FMT_UNORM8, seed_volumes=1,
+                     seed_weather=2, threads=0) -> Volumes:
+    return Volumes(noise(0, lf_dim, fmt, seed_volumes, threads=threads),
+                   noise(1, hf_dim, fmt, seed_volumes, threads=threads),
+                   noise(2, curl_dim, A.MTC_FMT_UNORM8, seed_volumes, threads=threads),
+                   noise(3, weather_dim, A.MTC_FMT_UNORM8, seed_weather, threads=threads),
+                   lf_dim, hf_dim, curl_dim, weather_dim, fmt)
+
+
+def select_pixels(W: int, H: int, frame: int) -> np.ndarray:
+    xy = np.empty(((W + 3) // 4) * ((H + 3) // 4) * 2, dtype=np.int32)
+    n = lib().orc_select_pixels(W, H, frame, xy.ctypes.data)
+    return xy[: 2 * n].reshape(n, 2)
+
+
+def march_pixels(vol: Volumes, cam, time, sun, params, W, H, xy: np.ndarray, precision=32, threads=0):
+    """Returns (rgba[n,4], mask[n,4], steps[n], counters dict), all float64/uint64."""
+    xy = np.ascontiguousarray(xy, dtype=np.int32).reshape(-1, 2)
+    n = xy.shape[0]
+    rgba = np.empty((n, 4), dtype=np.float64)
+    mask = np.empty((n, 4), dtype=np.float64)
+    steps = np.empty(n, dtype=np.uint64)
+    cnt = np.zeros(8, dtype=np.uint64)
+    rc = lib().orc_march_pixels(*vol.ptrs(), C.byref(vol.desc), C.byref(cam), C.byref(time), C.byref(sun),
+                                C.byref(params), W, H, xy.ctypes.data, n, precision, rgba.ctypes.data,
+                                mask.ctypes.data, steps.ctypes.data, cnt.ctypes.data, threads)
+    if rc:
+        raise ValueError("orc_march_pixels rejected its arguments")
+    return rgba, mask, steps, dict(zip(COUNTER_NAMES, (int(c) for c in cnt)))
+
+
+def frame(vol: Volumes, cam, old, time, sun, params, W, H, prev=None, precision=32, full=False, threads=0,
+          mask=None):
+    """One CPU frame: reprojection of prev (if given) then the march.
+    Returns (rgba[H,W,4] float32, mask[H,W,4] float32, counters)."""
+    cur = np.zeros((H, W, 4), dtype=np.float32)
+    m = np.zeros((H, W, 4), dtype=np.float32) if mask is None else np.ascontiguousarray(mask, dtype=np.float32).copy()
+    cnt = np.zeros(8, dtype=np.uint64)
+    prev_p = None
+    if prev is not None:
+        prev = np.ascontiguousarray(prev, dtype=np.float32)
+        prev_p = prev.ctypes.data
+    rc = lib().orc_frame(*vol.ptrs(), C.byref(vol.desc), C.byref(cam), C.byref(old) if old is not None else None,
+                         C.byref(time), C.byref(sun), C.byref(params), W, H, precision, int(full), prev_p,
+                         cur.ctypes.data, m.ctypes.data, cnt.ctypes.data, threads)
+    if rc:
+        raise ValueError("orc_frame rejected its arguments")
+    return cur, m, dict(zip(COUNTER_NAMES, (int(c) for c in cnt)))
+
+
+def reprojection_indices(cam, old, time, params, W, H, precision=32, threads=0) -> np.ndarray:
+    N = int(params.motion_blur_samples)
+    idx = np.empty(W * H * N, dtype=np.int32)
+    rc = lib().orc_reprojection_indices(C.byref(cam), C.byref(old), C.byref(time), C.byref(params), W, H, precision,
+                                        idx.ctypes.data, threads)
+    if rc:
+        raise ValueError("orc_reprojection_indices rejected its arguments")
+    return idx.reshape(H, W, N)
+
+
+def encode_float_rgba(v: float, precision=32):
+    if precision == 64:
+        out = (C.c_double * 4)()
+        lib().orc_encode_float_rgba64(v, out)
+    else:
+        out = (C.c_float * 4)()
+        lib().orc_encode_float_rgba(v, out)
+    return np.array(list(out))
+
+
+def hg(c, g, precision=32):
+    return lib().orc_hg(c, g, precision)
+
+
+def remap(v, a, b, c, d, kind=0):
+    return lib().orc_remap(v, a, b, c, d, kind)
+
+
+def smoothstep(e0, e1, x):
+    return lib().orc_smoothstep(e0, e1, x)
+
+
+def ray_sphere(o, d, c, R, precision=32):
+    out = (C.c_double * 5)()
+    lib().orc_ray_sphere(_d3(o), _d3(d), _d3(c), float(R), precision, out)
+    return bool(out[0]), np.array(out[1:4]), float(out[4])
+
+
+def sample3d(vol: np.ndarray, fmt: int, n: int, s, precision=32):
+    out = (C.c_double * 4)()
+    v = np.ascontiguousarray(vol)
+    lib().orc_sample3d(v.ctypes.data, fmt, n, _d3(s), precision, out)
+    return np.array(list(out))
+
+
+def sample2d(m: np.ndarray, n: int, sx, sy, precision=32):
+    out = (C.c_double * 4)()
+    v = np.ascontiguousarray(m, dtype=np.uint8)
+    lib().orc_sample2d(v.ctypes.data, n, float(sx), float(sy), precision, out)
+    return np.array(list(out))
+
+
+def atmosphere(sun, params, direction, origin=(0.0, 0.0, 0.0), precision=32):
+    out = (C.c_double * 3)()
+    lib().orc_atmosphere(C.byref(sun), C.byref(params), _d3(direction), _d3(origin), precision, out)
+    return np.array(list(out))
+
+
+def to_half(f: float) -> int:
+    return int(lib().orc_to_half(f))

@@ -1,0 +1,294 @@
+// oracle_capi.cpp -- C entry points of the CPU oracle (loaded by the tests
+// and by bench.py's cpu_baseline through ctypes).
+//
+// TEST INFRASTRUCTURE ONLY: the correctness oracle and CPU baseline of the
+// cloud compute stage.  Never linked into meteoros_amd.
+#include <omp.h>
+
+#include <cstdint>
+#include <cstring>
+#include <vector>
+
+#include "cloud_oracle.hpp"
+#include "noise_oracle.hpp"
+
+using orc::Counters;
+using orc::Frame;
+using orc::Volumes;
+
+static Volumes make_volumes(const void* lf, const void* hf, const uint8_t* curl, const uint8_t* weather,
+                            const mtc_volume_desc* vd) {
+  Volumes v;
+  v.lf = lf; v.lf_dim = vd->lf_dim;
+  v.hf = hf; v.hf_dim = vd->hf_dim;
+  v.curl = curl; v.curl_dim = vd->curl_dim;
+  v.weather = weather; v.weather_dim = vd->weather_dim;
+  v.format = vd->format;
+  return v;
+}
+
+static void counters_out(const Counters& c, uint64_t* out) {
+  if (!out) return;
+  out[0] = c.rays_marched; out[1] = c.rays_ocean; out[2] = c.rays_sky; out[3] = c.primary_steps;
+  out[4] = c.cloud_steps; out[5] = c.cone_samples; out[6] = c.cone_erosions; out[7] = c.early_exits;
+}
+
+template <class T>
+static void march_list(const Frame<T>& f, const Volumes& vol, const int32_t* xy, int64_t n, int pixelID,
+                       double* rgba, double* mask, uint64_t* steps, uint64_t* counters, int threads) {
+  Counters total;
+  if (threads > 0) omp_set_num_threads(threads);
+#pragma omp parallel
+  {
+    Counters local;
+#pragma omp for schedule(dynamic, 64)
+    for (int64_t i = 0; i < n; ++i) {
+      T c[4], m[4];
+      uint64_t before = local.primary_steps;
+      orc::march_pixel<T>(f, vol, xy[2 * i], xy[2 * i + 1], pixelID, c, m, local);
+      if (steps) steps[i] = local.primary_steps - before;
+      for (int k = 0; k < 4; ++k) {
+        if (rgba) rgba[4 * i + k] = (double)c[k];
+        if (mask) mask[4 * i + k] = (double)m[k];
+      }
+    }
+#pragma omp critical
+    total.add(local);
+  }
+  counters_out(total, counters);
+}
+
+extern "C" {
+
+// March an explicit list of pixels (x, y pairs) with the frame's jitter
+// phase (time->frame_count_mod16).  precision = 32 or 64.  Outputs are
+// double so the FP64 twin keeps its bits; rgba/mask/steps may be NULL.
+// counters (8 x u64): rays_marched, rays_ocean, rays_sky, primary_steps,
+// cloud_steps, cone_samples, cone_erosions, early_exits.
+int orc_march_pixels(const void* lf, const void* hf, const uint8_t* curl, const uint8_t* weather,
+                     const mtc_volume_desc* vd, const mtc_camera* cam, const mtc_time* tm, const mtc_sun* sun,
+                     const mtc_cloud_params* p, int W, int H, const int32_t* xy, int64_t n, int precision,
+                     double* rgba, double* mask, uint64_t* steps, uint64_t* counters, int threads) {
+  if (!lf || !hf || !curl || !weather || !vd || !cam || !tm || !sun || !p || !xy || W <= 0 || H <= 0) return 1;
+  Volumes vol = make_volumes(lf, hf, curl, weather, vd);
+  int pid = tm->frame_count_mod16;
+  if (precision == 64) {
+    Frame<double> f = orc::make_frame<double>(*cam, *tm, *sun, *p, W, H);
+    march_list<double>(f, vol, xy, n, pid, rgba, mask, steps, counters, threads);
+  } else {
+    Frame<float> f = orc::make_frame<float>(*cam, *tm, *sun, *p, W, H);
+    march_list<float>(f, vol, xy, n, pid, rgba, mask, steps, counters, threads);
+  }
+  return 0;
+}
+
+// The 4x4 selection of cloudRayMarch.comp:695-704 for one frame: writes the
+// list of chosen pixels (x, y) into xy (capacity ceil(W/4)*ceil(H/4)) and
+// returns their number.
+int64_t orc_select_pixels(int W, int H, int frame_mod16, int32_t* xy) {
+  int pX = frame_mod16 / 4, pY = frame_mod16 % 4;
+  int64_t n = 0;
+  int gw = (W + 3) / 4, gh = (H + 3) / 4;
+  for (int gy = 0; gy < gh; ++gy)
+    for (int gx = 0; gx < gw; ++gx) {
+      int x = gx * 4 + pX, y = gy * 4 + pY;
+      if (x < W && y < H) {
+        xy[2 * n] = x;
+        xy[2 * n + 1] = y;
+        ++n;
+      }
+    }
+  return n;
+}
+
+// One frame of the stage on the CPU: reprojection of `prev` into `cur`
+// (skipped when prev is NULL), then the march of the selected pixels (all
+// pixels when full != 0) written into cur / mask.  FP32 (precision 32) or
+// FP64 (precision 64) arithmetic; images stay RGBA32F.
+int orc_frame(const void* lf, const void* hf, const uint8_t* curl, const uint8_t* weather, const mtc_volume_desc* vd,
+              const mtc_camera* cam, const mtc_camera* old, const mtc_time* tm, const mtc_sun* sun,
+              const mtc_cloud_params* p, int W, int H, int precision, int full, const float* prev, float* cur,
+              float* mask, uint64_t* counters, int threads) {
+  if (!cur || !mask || !cam || !tm || !p || W <= 0 || H <= 0) return 1;
+  if (threads > 0) omp_set_num_threads(threads);
+  if (prev) {
+    if (!old) return 1;
+    if (precision == 64) {
+      Frame<double> f = orc::make_frame<double>(*cam, *tm, *sun, *p, W, H);
+      double ov[16];
+      for (int i = 0; i < 16; ++i) ov[i] = old->view[i];
+#pragma omp parallel for schedule(static)
+      for (int y = 0; y < H; ++y)
+        for (int x = 0; x < W; ++x) orc::reproject_pixel<double>(f, ov, prev, x, y, cur + ((size_t)y * W + x) * 4, nullptr);
+    } else {
+      Frame<float> f = orc::make_frame<float>(*cam, *tm, *sun, *p, W, H);
+      float ov[16];
+      for (int i = 0; i < 16; ++i) ov[i] = old->view[i];
+#pragma omp parallel for schedule(static)
+      for (int y = 0; y < H; ++y)
+        for (int x = 0; x < W; ++x) orc::reproject_pixel<float>(f, ov, prev, x, y, cur + ((size_t)y * W + x) * 4, nullptr);
+    }
+  }
+  std::vector<int32_t> xy;
+  int64_t n;
+  if (full) {
+    xy.resize((size_t)W * H * 2);
+    n = 0;
+    for (int y = 0; y < H; ++y)
+      for (int x = 0; x < W; ++x) { xy[2 * n] = x; xy[2 * n + 1] = y; ++n; }
+  } else {
+    xy.resize((size_t)((W + 3) / 4) * ((H + 3) / 4) * 2);
+    n = orc_select_pixels(W, H, tm->frame_count_mod16, xy.data());
+  }
+  std::vector<double> c((size_t)n * 4), m((size_t)n * 4);
+  int rc = orc_march_pixels(lf, hf, curl, weather, vd, cam, tm, sun, p, W, H, xy.data(), n, precision, c.data(),
+                            m.data(), nullptr, counters, threads);
+  if (rc) return rc;
+  for (int64_t i = 0; i < n; ++i) {
+    size_t o = ((size_t)xy[2 * i + 1] * W + xy[2 * i]) * 4;
+    for (int k = 0; k < 4; ++k) {
+      cur[o + k] = (float)c[4 * i + k];
+      mask[o + k] = (float)m[4 * i + k];
+    }
+  }
+  return 0;
+}
+
+// Reprojection source indices: idx holds W*H*motion_blur_samples int32.
+int orc_reprojection_indices(const mtc_camera* cam, const mtc_camera* old, const mtc_time* tm,
+                             const mtc_cloud_params* p, int W, int H, int precision, int32_t* idx, int threads) {
+  if (!cam || !old || !tm || !p || !idx || W <= 0 || H <= 0) return 1;
+  if (threads > 0) omp_set_num_threads(threads);
+  mtc_sun sun;
+  std::memset(&sun, 0, sizeof(sun));
+  const int N = p->motion_blur_samples;
+  if (precision == 64) {
+    Frame<double> f = orc::make_frame<double>(*cam, *tm, sun, *p, W, H);
+    double ov[16];
+    for (int i = 0; i < 16; ++i) ov[i] = old->view[i];
+#pragma omp parallel for schedule(static)
+    for (int y = 0; y < H; ++y)
+      for (int x = 0; x < W; ++x) {
+        float o[4];
+        orc::reproject_pixel<double>(f, ov, nullptr, x, y, o, idx + ((size_t)y * W + x) * N);
+      }
+  } else {
+    Frame<float> f = orc::make_frame<float>(*cam, *tm, sun, *p, W, H);
+    float ov[16];
+    for (int i = 0; i < 16; ++i) ov[i] = old->view[i];
+#pragma omp parallel for schedule(static)
+    for (int y = 0; y < H; ++y)
+      for (int x = 0; x < W; ++x) {
+        float o[4];
+        orc::reproject_pixel<float>(f, ov, nullptr, x, y, o, idx + ((size_t)y * W + x) * N);
+      }
+  }
+  return 0;
+}
+
+// ---------------------------------------------------------------------------
+// small pieces exposed for the oracle's own pinning tests
+// ---------------------------------------------------------------------------
+void orc_encode_float_rgba(float v, float out[4]) { orc::encode_float_rgba<float>(v, out); }
+void orc_encode_float_rgba64(double v, double out[4]) { orc::encode_float_rgba<double>(v, out); }
+double orc_hg(double c, double g, int precision) {
+  return precision == 64 ? orc::hg<double>(c, g) : (double)orc::hg<float>((float)c, (float)g);
+}
+double orc_remap(double v, double a, double b, double c, double d, int kind) {
+  float fv = (float)v, fa = (float)a, fb = (float)b, fc = (float)c, fd = (float)d;
+  if (kind == 0) return orc::remap<float>(fv, fa, fb, fc, fd);
+  if (kind == 1) return orc::remapClamped<float>(fv, fa, fb, fc, fd);
+  return orc::remapClampedBeforeAndAfter<float>(fv, fa, fb, fc, fd);
+}
+double orc_smoothstep(double e0, double e1, double x) { return orc::smoothstepT<float>((float)e0, (float)e1, (float)x); }
+// ray-sphere: out = (valid, px, py, pz, t)
+void orc_ray_sphere(const double o[3], const double d[3], const double c[3], double R, int precision, double out[5]) {
+  if (precision == 64) {
+    auto is = orc::ray_sphere<double>(orc::V(o[0], o[1], o[2]), orc::V(d[0], d[1], d[2]), orc::V(c[0], c[1], c[2]), R);
+    out[0] = is.valid; out[1] = is.point.x; out[2] = is.point.y; out[3] = is.point.z; out[4] = is.t;
+  } else {
+    auto is = orc::ray_sphere<float>(orc::V((float)o[0], (float)o[1], (float)o[2]),
+                                     orc::V((float)d[0], (float)d[1], (float)d[2]),
+                                     orc::V((float)c[0], (float)c[1], (float)c[2]), (float)R);
+    out[0] = is.valid; out[1] = is.point.x; out[2] = is.point.y; out[3] = is.point.z; out[4] = is.t;
+  }
+}
+// trilinear sample of a volume at normalised s; out = 4 channels (double)
+void orc_sample3d(const void* base, int fmt, int N, const double s[3], int precision, double out[4]) {
+  if (precision == 64) {
+    auto c = orc::sample3d<double>(base, fmt, N, orc::V(s[0], s[1], s[2]));
+    out[0] = c.r; out[1] = c.g; out[2] = c.b; out[3] = c.a;
+  } else {
+    auto c = orc::sample3d<float>(base, fmt, N, orc::V((float)s[0], (float)s[1], (float)s[2]));
+    out[0] = c.r; out[1] = c.g; out[2] = c.b; out[3] = c.a;
+  }
+}
+void orc_sample2d(const uint8_t* base, int N, double sx, double sy, int precision, double out[4]) {
+  if (precision == 64) {
+    auto c = orc::sample2d<double>(base, N, sx, sy);
+    out[0] = c.r; out[1] = c.g; out[2] = c.b; out[3] = c.a;
+  } else {
+    auto c = orc::sample2d<float>(base, N, (float)sx, (float)sy);
+    out[0] = c.r; out[1] = c.g; out[2] = c.b; out[3] = c.a;
+  }
+}
+// background sky colour for a direction (Preetham, cloudRayMarch.comp:427-467)
+void orc_atmosphere(const mtc_sun* sun, const mtc_cloud_params* p, const double dir[3], const double origin[3],
+                    int precision, double out[3]) {
+  mtc_camera cam;
+  std::memset(&cam, 0, sizeof(cam));
+  mtc_time tm;
+  std::memset(&tm, 0, sizeof(tm));
+  if (precision == 64) {
+    Frame<double> f = orc::make_frame<double>(cam, tm, *sun, *p, 1, 1);
+    auto o = orc::V(origin[0], origin[1], origin[2]);
+    auto c = orc::atmosphere_color<double>(f, orc::V(dir[0], dir[1], dir[2]), f.sky_sun - o, f.sun_intensity);
+    out[0] = c.x; out[1] = c.y; out[2] = c.z;
+  } else {
+    Frame<float> f = orc::make_frame<float>(cam, tm, *sun, *p, 1, 1);
+    auto o = orc::V((float)origin[0], (float)origin[1], (float)origin[2]);
+    auto c = orc::atmosphere_color<float>(f, orc::V((float)dir[0], (float)dir[1], (float)dir[2]), f.sky_sun - o,
+                                          f.sun_intensity);
+    out[0] = c.x; out[1] = c.y; out[2] = c.z;
+  }
+}
+
+// ---------------------------------------------------------------------------
+// noise generator (specification of kernel a)
+// ---------------------------------------------------------------------------
+// kind: 0 = LF (N^3), 1 = HF (N^3), 2 = curl (N^2), 3 = weather (N^2).
+// fmt: MTC_FMT_UNORM8 (4 bytes/texel) or MTC_FMT_FP16 (4 halves/texel;
+// LF/HF only).  z0/nz select a z-slab of a 3D volume (nz <= 0: all); the
+// output holds only that slab.
+int orc_noise_generate(int kind, int N, int fmt, uint32_t seed, int z0, int nz, void* out, int threads) {
+  if (!out || N <= 0) return 1;
+  if (threads > 0) omp_set_num_threads(threads);
+  int depth = (kind <= 1) ? N : 1;
+  if (nz <= 0) { z0 = 0; nz = depth; }
+  if (z0 < 0 || z0 + nz > depth) return 1;
+  if (kind >= 2 && fmt != MTC_FMT_UNORM8) return 1;
+#pragma omp parallel for schedule(dynamic, 1) collapse(2)
+  for (int zz = 0; zz < nz; ++zz)
+    for (int y = 0; y < N; ++y) {
+      int z = z0 + zz;
+      for (int x = 0; x < N; ++x) {
+        float v[4];
+        if (kind == 0) orc_noise::lf_texel(x, y, z, N, seed, v);
+        else if (kind == 1) orc_noise::hf_texel(x, y, z, N, seed, v);
+        else if (kind == 2) orc_noise::curl_texel(x, y, N, seed, v);
+        else orc_noise::weather_texel(x, y, N, seed, v);
+        size_t o = (((size_t)zz * N + y) * N + x) * 4;
+        if (fmt == MTC_FMT_FP16) {
+          uint16_t* h = (uint16_t*)out;
+          for (int k = 0; k < 4; ++k) h[o + k] = orc_noise::to_half(orc_noise::clamp01(v[k]));
+        } else {
+          uint8_t* b = (uint8_t*)out;
+          for (int k = 0; k < 4; ++k) b[o + k] = orc_noise::to_unorm8(v[k]);
+        }
+      }
+    }
+  return 0;
+}
+uint16_t orc_to_half(float f) { return orc_noise::to_half(f); }
+
+}  // extern "C"

@@ -1,0 +1,108 @@
+"""The C-ABI library loads, exports every entry point of
+include/meteoros_cloud.h, keeps the std140 uniform layouts of the reference
+(camera.h:12-18, Scene.h:12-21, Sky.h:9-15), and rejects bad input on the
+host before any device work."""
+import ctypes as C
+import os
+import re
+
+import pytest
+
+import meteoros_amd as M
+from meteoros_amd import _abi as A
+
+HEADER = os.path.join(os.path.dirname(os.path.dirname(os.path.abspath(__file__))), "include", "meteoros_cloud.h")
+
+
+def declared_functions():
+    text = open(HEADER).read()
+    text = re.sub(r"/\*.*?\*/", "", text, flags=re.S)
+    return sorted(set(re.findall(r"\b(mtc_[a-z0-9_]+)\s*\(", text)))
+
+
+def test_library_exports_every_declared_symbol():
+    lib = M.lib()
+    names = declared_functions()
+    assert len(names) >= 25
+    missing = [n for n in names if not hasattr(lib, n)]
+    assert not missing, missing
+    bound = {s[0] for s in A.SIGNATURES}
+    assert set(names) <= bound, set(names) - bound
+
+
+def test_std140_layouts():
+    assert C.sizeof(A.MtcCamera) == 152
+    assert A.MtcCamera.proj.offset == 64 and A.MtcCamera.eye.offset == 128 and A.MtcCamera.tan_fov_by2.offset == 144
+    assert C.sizeof(A.MtcTime) == 76
+    assert A.MtcTime.halton2.offset == 16 and A.MtcTime.halton4.offset == 48
+    assert A.MtcTime.time.offset == 64 and A.MtcTime.frame_count_mod16.offset == 72
+    assert C.sizeof(A.MtcSun) == 52 and A.MtcSun.sun_intensity.offset == 48
+
+
+def test_version_and_device_count_never_fail():
+    assert b"gfx950" in M.lib().mtc_version()
+    assert M.device_count() >= 0
+
+
+def _desc(**kw):
+    d = A.MtcDesc()
+    d.width, d.height = kw.get("width", 64), kw.get("height", 32)
+    d.volumes = A.MtcVolumeDesc(*kw.get("vol", (32, 16, 32, 64, A.MTC_FMT_UNORM8)))
+    d.kernel = kw.get("kernel", A.MTC_KERNEL_PLAIN)
+    d.flags = kw.get("flags", 0)
+    return d
+
+
+@pytest.mark.parametrize("kw", [
+    dict(width=0), dict(height=-3), dict(width=20000),
+    dict(vol=(30, 16, 32, 64, 0)), dict(vol=(32, 16, 32, 64, 7)), dict(kernel=17), dict(flags=64),
+])
+def test_create_rejects_invalid_descriptions(kw):
+    ctx = C.c_void_p()
+    rc = M.lib().mtc_create(C.byref(_desc(**kw)), C.byref(ctx))
+    assert rc == A.MTC_ERR_INVALID_ARG
+    assert not ctx
+    assert M.lib().mtc_last_error()
+
+
+def test_create_without_device_is_an_error_not_a_crash():
+    if M.device_count() > 0:
+        pytest.skip("a device is present")
+    ctx = C.c_void_p()
+    rc = M.lib().mtc_create(C.byref(_desc()), C.byref(ctx))
+    assert rc == A.MTC_ERR_HIP
+    assert b"device" in M.lib().mtc_last_error()
+
+
+def test_null_context_calls_are_rejected():
+    lib = M.lib()
+    assert lib.mtc_synchronize(None) == A.MTC_ERR_INVALID_ARG
+    assert lib.mtc_read_output(None, None, None, 0) == A.MTC_ERR_INVALID_ARG
+    assert lib.mtc_frame(None, None, None, None, None, None, None) == A.MTC_ERR_INVALID_ARG
+    assert lib.mtc_destroy(None) == A.MTC_OK
+
+
+def test_camera_helpers_validate():
+    lib = M.lib()
+    st = A.MtcCameraState()
+    e = (C.c_float * 3)(0, 0, 0)
+    assert lib.mtc_camera_init(C.byref(st), e, e, 64, 64, 45.0, 0.1, 100.0) == A.MTC_ERR_INVALID_ARG
+    r = (C.c_float * 3)(0, 0, -1)
+    assert lib.mtc_camera_init(C.byref(st), e, r, 64, 64, 190.0, 0.1, 100.0) == A.MTC_ERR_INVALID_ARG
+    assert lib.mtc_camera_init(C.byref(st), e, r, 64, 64, 45.0, 0.1, 100.0) == A.MTC_OK
+    t = A.MtcTime()
+    assert lib.mtc_time_init(C.byref(t)) == A.MTC_OK
+    assert lib.mtc_time_advance(C.byref(t), float("nan")) == A.MTC_ERR_INVALID_ARG
+    assert lib.mtc_time_advance(C.byref(t), -1.0) == A.MTC_ERR_INVALID_ARG
+    p, s = A.MtcCloudParams(), A.MtcSun()
+    assert lib.mtc_params_preset(0, C.byref(p), C.byref(s)) == A.MTC_ERR_INVALID_ARG
+    assert lib.mtc_params_preset(6, C.byref(p), C.byref(s)) == A.MTC_ERR_INVALID_ARG
+    for k in range(1, 6):
+        assert lib.mtc_params_preset(k, C.byref(p), C.byref(s)) == A.MTC_OK
+
+
+def test_missing_library_fails_loudly(monkeypatch, tmp_path):
+    monkeypatch.setattr(A, "_lib", None)
+    monkeypatch.setattr(A, "LIB_PATH", str(tmp_path / "nope.so"))
+    with pytest.raises(ImportError):
+        A.lib()

@@ -1,0 +1,208 @@
+"""Parity of the HIP kernels (through the C ABI) with the CPU oracle.
+
+Bars: noise volumes, 4x4 selection, step counts and reprojection tap
+indices bit-exact; colours and god-ray mask of the plain FP32 kernel within
+1e-3 relative per value (floor 1e-3 of the frame's peak).
+"""
+import numpy as np
+import pytest
+
+import meteoros_amd as M
+from meteoros_amd import _abi as A
+from oracle import oracle as O
+from tests.common import ground_camera, volumes, within_rel
+
+pytestmark = pytest.mark.gpu
+
+SMALL = (32, 16, 32, 64)
+FULL = (128, 32, 128, 512)
+
+
+def _stage(W, H, dims=FULL, fmt=A.MTC_FMT_UNORM8, flags=0, kernel=A.MTC_KERNEL_PLAIN, vol=None):
+    st = M.CloudStage(W, H, M.VolumeSpec(*dims, fmt), kernel=kernel, flags=flags)
+    v = vol if vol is not None else volumes(*dims, fmt=fmt)
+    st.set_volumes(v.lf, v.hf, v.curl, v.weather)
+    return st, v
+
+
+def _time(frame=1, total=1.0 / 60.0):
+    t = M.Time()
+    t.ubo.frame_count_mod16 = frame
+    t.ubo.time[0] = 1.0 / 60.0
+    t.ubo.time[1] = total
+    return t
+
+
+@pytest.mark.parametrize("dims,fmt", [(SMALL, 0), (FULL, 0), ((64, 32, 64, 128), 1)])
+def test_noise_generation_is_bit_exact(dims, fmt):
+    with M.CloudStage(16, 16, M.VolumeSpec(*dims, fmt)) as st:
+        st.generate_volumes(1, 2)
+        lf, hf, curl, weather = st.get_volumes()
+    ref = O.generate_volumes(*dims, fmt=fmt, seed_volumes=1, seed_weather=2)
+    assert np.array_equal(lf, ref.lf)
+    assert np.array_equal(hf, ref.hf)
+    assert np.array_equal(curl, ref.curl)
+    assert np.array_equal(weather, ref.weather)
+
+
+def _compare_selected(rgba, mask, W, H, vol, cam, tm, sun, params):
+    xy = O.select_pixels(W, H, tm.frame_count_mod16)
+    ref, refm, steps, cnt = O.march_pixels(vol, cam, tm, sun, params, W, H, xy, 32)
+    got = rgba[xy[:, 1], xy[:, 0]]
+    gotm = mask[xy[:, 1], xy[:, 0]]
+    bad = within_rel(got, ref)
+    badm = within_rel(gotm, refm, floor_frac=1.0)
+    assert not bad.any(), f"{int(bad.sum())} colour values off; worst rel {np.max(np.abs(got - ref) / np.maximum(np.abs(ref), 1e-6))}"
+    assert not badm.any(), f"{int(badm.sum())} mask values off"
+    return xy, cnt
+
+
+@pytest.mark.parametrize("W,H,preset,pitch", [(480, 270, 1, 20.0), (97, 61, 1, 10.0), (256, 144, 2, 30.0)])
+def test_plain_march_matches_oracle(W, H, preset, pitch):
+    params, sun = M.preset_params(preset)
+    cam = ground_camera(W, H, pitch).ubo()
+    for frame in (1, 6, 15):
+        tm = _time(frame, frame / 60.0).ubo
+        st, v = _stage(W, H, flags=A.MTC_FLAG_COUNTERS)
+        with st:
+            st.frame(cam, cam, tm, sun, params)
+            rgba, mask = st.read_output()
+            c = st.counters()
+        xy, cnt = _compare_selected(rgba, mask, W, H, v, cam, tm, sun, params)
+        for k in O.COUNTER_NAMES:
+            assert c[k] == cnt[k], (k, c[k], cnt[k])
+        assert c["rays_marched"] + c["rays_ocean"] + c["rays_sky"] == xy.shape[0]
+
+
+def test_plain_march_c2_frame_1080p():
+    W, H = 1920, 1080
+    params, sun = M.preset_params(2)
+    cam = ground_camera(W, H, 20.0).ubo()
+    tm = _time(1).ubo
+    st, v = _stage(W, H, flags=A.MTC_FLAG_COUNTERS)
+    with st:
+        st.frame(cam, cam, tm, sun, params)
+        rgba, mask = st.read_output()
+        c = st.counters()
+    xy, cnt = _compare_selected(rgba, mask, W, H, v, cam, tm, sun, params)
+    assert c["primary_steps"] == cnt["primary_steps"]
+    assert xy.shape[0] == 129600
+
+
+def test_reference_preset_matches_oracle():
+    W, H = 160, 120
+    params, sun = M.reference_params()
+    cam = M.Camera((0.0, 0.0, 2.0), (0.0, 0.5, 1.0), W, H, 45.0).ubo()
+    tm = _time(3, 0.5).ubo
+    st, v = _stage(W, H)
+    with st:
+        st.frame(cam, cam, tm, sun, params)
+        rgba, mask = st.read_output()
+    _compare_selected(rgba, mask, W, H, v, cam, tm, sun, params)
+
+
+def test_full_march_every_pixel():
+    W, H = 120, 68
+    params, sun = M.preset_params(1)
+    cam = ground_camera(W, H, 15.0).ubo()
+    tm = _time(4).ubo
+    st, v = _stage(W, H, dims=SMALL, vol=volumes(*SMALL))
+    with st:
+        st.march_full(cam, tm, sun, params)
+        rgba, mask = st.read_output()
+    ref, refm, _ = O.frame(v, cam, None, tm, sun, params, W, H, precision=32, full=True)
+    assert not within_rel(rgba, ref).any()
+    assert not within_rel(mask, refm, floor_frac=1.0).any()
+
+
+def test_ocean_and_empty_volume_edge_cases():
+    W, H = 64, 40
+    params, sun = M.preset_params(1)
+    v = volumes(*SMALL)
+    lf = np.zeros_like(v.lf).reshape(-1, 4)
+    lf[:, 1:] = 255  # zero density everywhere
+    empty = O.Volumes(lf.reshape(-1), np.zeros_like(v.hf), v.curl, v.weather, *SMALL)
+    for pitch, vol in ((-70.0, v), (25.0, empty)):
+        cam = ground_camera(W, H, pitch).ubo()
+        tm = _time(0).ubo
+        st, _ = _stage(W, H, dims=SMALL, vol=vol)
+        with st:
+            st.march_full(cam, tm, sun, params)
+            rgba, mask = st.read_output()
+        ref, refm, cnt = O.frame(vol, cam, None, tm, sun, params, W, H, precision=32, full=True)
+        assert not within_rel(rgba, ref).any()
+        assert np.array_equal(mask, refm)
+
+
+@pytest.mark.parametrize("W,H", [(480, 270), (1920, 1080)])
+def test_reprojection_indices_bit_exact(W, H):
+    params, sun = M.preset_params(2)
+    camera = ground_camera(W, H, 20.0)
+    old = camera.ubo()
+    camera.rotate_about_up(0.5)
+    camera.rotate_about_right(0.3)
+    cur = camera.ubo()
+    tm = _time(7).ubo
+    with M.CloudStage(W, H, M.VolumeSpec(*SMALL)) as st:
+        got = st.reprojection_indices(cur, old, tm, params)
+    ref = O.reprojection_indices(cur, old, tm, params, W, H, 32)
+    assert np.array_equal(got, ref), f"{int((got != ref).sum())} tap indices differ"
+
+
+def test_frame_sequence_with_reprojection_matches_oracle():
+    """Eight frames, yaw 0.5 deg/frame, 4x4 reprojection (the frame loop of
+    main.cpp:172-186): time -> frame -> old camera <- camera."""
+    W, H = 240, 136
+    params, sun = M.preset_params(2)
+    camera = ground_camera(W, H, 20.0)
+    time = M.Time()
+    st, v = _stage(W, H, dims=SMALL, vol=volumes(*SMALL))
+    old = camera.ubo()
+    prev = np.zeros((H, W, 4), np.float32)
+    mask_ref = np.zeros((H, W, 4), np.float32)
+    with st:
+        for k in range(8):
+            camera.rotate_about_up(0.5)
+            time.advance(1.0 / 60.0)
+            cur = camera.ubo()
+            st.frame(cur, old, time.ubo, sun, params)
+            rgba, mask = st.read_output()
+            ref, mask_ref, _ = O.frame(v, cur, old, time.ubo, sun, params, W, H, prev=prev, precision=32,
+                                       mask=mask_ref)
+            bad = within_rel(rgba, ref)
+            assert not bad.any(), (k, int(bad.sum()))
+            assert not within_rel(mask, mask_ref, floor_frac=1.0).any(), k
+            prev = ref
+            old = cur
+
+
+def test_history_and_error_paths():
+    W, H = 32, 24
+    params, sun = M.preset_params(1)
+    cam = ground_camera(W, H).ubo()
+    tm = _time(2).ubo
+    with M.CloudStage(W, H, M.VolumeSpec(*SMALL)) as st:
+        with pytest.raises(M.MtcError) as e:
+            st.frame(cam, cam, tm, sun, params)
+        assert e.value.code == A.MTC_ERR_STATE
+        st.generate_volumes(1, 2)
+        bad_t = _time(2).ubo
+        bad_t.frame_count_mod16 = 16
+        with pytest.raises(M.MtcError):
+            st.frame(cam, cam, bad_t, sun, params)
+        nan_cam = ground_camera(W, H).ubo()
+        nan_cam.view[3] = float("nan")
+        with pytest.raises(M.MtcError):
+            st.frame(nan_cam, cam, tm, sun, params)
+        hist = np.full((H, W, 4), 0.25, np.float32)
+        st.set_history(hist)
+        st.frame(cam, cam, tm, sun, params)
+        rgba, _ = st.read_output()
+        sel = np.zeros((H, W), bool)
+        xy = O.select_pixels(W, H, 2)
+        sel[xy[:, 1], xy[:, 0]] = True
+        assert np.allclose(rgba[~sel], 0.25, rtol=1e-6)
+        st.reset_history()
+        st.frame(cam, cam, tm, sun, params)
+        rgba, _ = st.read_output()
+        assert np.all(rgba[~sel] == 0.0)
