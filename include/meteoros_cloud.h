@@ -218,7 +218,8 @@ enum {
 
 enum {
   MTC_FLAG_COUNTERS = 1,     /* accumulate mtc_counters with device atomics  */
-  MTC_FLAG_FULL_MARCH = 2    /* march every pixel (no 4x4 selection)          */
+  MTC_FLAG_FULL_MARCH = 2,   /* march every pixel (no 4x4 selection)          */
+  MTC_FLAG_TIMING = 4        /* hipEvents around each kernel (mtc_get_timings) */
 };
 
 typedef struct mtc_desc {
@@ -244,6 +245,14 @@ typedef struct mtc_counters {
   uint64_t mlp_queries;      /* learned-light queries                 */
   uint64_t mlp_fallbacks;    /* learned-light queries sent to the cone */
 } mtc_counters;
+
+/* Kernel time accumulated with hipEvents on the launch stream
+ * (MTC_FLAG_TIMING), milliseconds summed over `frames` frames. */
+typedef struct mtc_timings {
+  uint64_t frames;
+  double reproject_ms;
+  double march_ms;
+} mtc_timings;
 
 typedef struct mtc_ctx mtc_ctx;
 
@@ -295,6 +304,11 @@ int mtc_reset_history(mtc_ctx* ctx);
 /* Counters accumulated since the last reset (needs MTC_FLAG_COUNTERS). */
 int mtc_get_counters(mtc_ctx* ctx, mtc_counters* out);
 int mtc_reset_counters(mtc_ctx* ctx);
+
+/* Kernel timings since the last reset (needs MTC_FLAG_TIMING; waits for
+ * the recorded events). */
+int mtc_get_timings(mtc_ctx* ctx, mtc_timings* out);
+int mtc_reset_timings(mtc_ctx* ctx);
 
 /* Reprojection source-texel map for testing: for every pixel and tap the
  * linear index y*W+x of the previous-frame texel the tap reads

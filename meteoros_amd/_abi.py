@@ -32,6 +32,7 @@ MTC_KERNEL_PRECISION = 2
 MTC_KERNEL_LEARNED = 3
 MTC_FLAG_COUNTERS = 1
 MTC_FLAG_FULL_MARCH = 2
+MTC_FLAG_TIMING = 4
 
 
 class MtcCamera(C.Structure):
@@ -94,6 +95,10 @@ class MtcCounters(C.Structure):
         "cone_erosions", "early_exits", "mlp_queries", "mlp_fallbacks")]
 
 
+class MtcTimings(C.Structure):
+    _fields_ = [("frames", C.c_uint64), ("reproject_ms", C.c_double), ("march_ms", C.c_double)]
+
+
 P = C.POINTER
 # (name, restype, argtypes) of every exported entry point of meteoros_cloud.h
 SIGNATURES = [
@@ -128,6 +133,8 @@ SIGNATURES = [
     ("mtc_reset_history", C.c_int, [C.c_void_p]),
     ("mtc_get_counters", C.c_int, [C.c_void_p, P(MtcCounters)]),
     ("mtc_reset_counters", C.c_int, [C.c_void_p]),
+    ("mtc_get_timings", C.c_int, [C.c_void_p, P(MtcTimings)]),
+    ("mtc_reset_timings", C.c_int, [C.c_void_p]),
     ("mtc_reprojection_indices", C.c_int, [C.c_void_p, P(MtcCamera), P(MtcCamera), P(MtcTime), P(MtcCloudParams),
                                            C.c_void_p, C.c_size_t]),
 ]

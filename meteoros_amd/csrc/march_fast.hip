@@ -1,0 +1,555 @@
+// march_fast.hip -- optimised cloud ray-march kernels (kernel b, fast and
+// precision variants) and the footprint-packed resources they read.
+//
+// Same algorithm as cloudRayMarch.comp (:565-826) and the plain kernel; the
+// differences are all in how the work maps onto gfx950:
+//
+//  * Footprint-packed texels.  Every trilinear (bilinear) fetch of the
+//    plain kernel issues 8 (4) dword loads.  Here each texel of a derived
+//    volume stores its whole 2x2 footprint in one 16-byte entry, so an LF
+//    sample is two dwordx4 loads (z0 and z1 slices), a curl / weather / HF
+//    sample one.  Only the channels the density needs are kept, already
+//    combined: LF -> (r, fbm = .625g + .25b + .125a) as half2 (filtering is
+//    linear, so filtering the combined channel equals combining the
+//    filtered channels, :505-507); HF -> fbm (:552-554); curl -> (r, g);
+//    weather -> (coverage, type).  Built once per volume upload.
+//  * Erosion hoisted out of the cone.  The six cone erosions of a step
+//    read the curl and HF volumes at the primary step's skewed point with
+//    the primary height (:665), so their high-frequency modifier equals
+//    the primary erosion's; it is fetched once per cloud step and each
+//    erosion is one remap.
+//  * Loop invariants (length(start - eye), the step's normalisation,
+//    reciprocals of the constant denominators) are hoisted; the single
+//    exp of GetLightEnergy is shared by its two identical terms; pow runs
+//    as exp2(y * log2(x)) on the transcendental unit.
+//  * t_k = start + k * step is evaluated directly (one rounding) instead of
+//    by repeated addition.
+//  * Variant 1 (precision): filtering in packed FP16 (v_pk_fma_f16), FP16
+//    volumes, the shell quadratic in FP64; transmittance and radiance stay
+//    FP32.
+#include "march_common.h"
+
+typedef _Float16 h2v __attribute__((ext_vector_type(2)));
+
+namespace mtc {
+namespace fast {
+
+__device__ __forceinline__ h2v as_h2(uint32_t v) { return __builtin_bit_cast(h2v, v); }
+__device__ __forceinline__ uint32_t pack_h2(float a, float b) {
+  h2v h = {(_Float16)a, (_Float16)b};
+  return __builtin_bit_cast(uint32_t, h);
+}
+__device__ __forceinline__ float2 f2(uint32_t v) {
+  h2v h = as_h2(v);
+  return make_float2((float)h.x, (float)h.y);
+}
+__device__ __forceinline__ float2 lerp2(float2 p, float2 q, float w) {
+  return make_float2(__fmaf_rn(w, q.x - p.x, p.x), __fmaf_rn(w, q.y - p.y, p.y));
+}
+__device__ __forceinline__ h2v lerp2h(h2v p, h2v q, h2v w) { return __builtin_elementwise_fma(w, q - p, p); }
+
+struct Coord {
+  int i0;        // linear index of the (x0, y0, z0) footprint entry
+  int dz;        // index offset of the z1 slice (wraps)
+  float ax, ay, az;
+};
+
+__device__ __forceinline__ Coord coord3(int sh, f3 s) {
+  const float N = (float)(1 << sh);
+  const int m = (1 << sh) - 1;
+  float ux = s.x * N - 0.5f, uy = s.y * N - 0.5f, uz = s.z * N - 0.5f;
+  float fx = floorf(ux), fy = floorf(uy), fz = floorf(uz);
+  Coord c;
+  c.ax = ux - fx;
+  c.ay = uy - fy;
+  c.az = uz - fz;
+  int x0 = (int)fx & m, y0 = (int)fy & m, z0 = (int)fz & m;
+  int z1 = (z0 + 1) & m;
+  c.i0 = (((z0 << sh) + y0) << sh) + x0;
+  c.dz = (z1 - z0) << (2 * sh);
+  return c;
+}
+
+// LF: (r, fbm) trilinear from two footprint entries
+template <bool F16>
+__device__ __forceinline__ float2 sample_lf(const uint4* __restrict__ q, int sh, f3 s) {
+  Coord c = coord3(sh, s);
+  uint4 a = q[c.i0];
+  uint4 b = q[c.i0 + c.dz];
+  if (F16) {
+    h2v wx = {(_Float16)c.ax, (_Float16)c.ax}, wy = {(_Float16)c.ay, (_Float16)c.ay},
+        wz = {(_Float16)c.az, (_Float16)c.az};
+    h2v a0 = lerp2h(as_h2(a.x), as_h2(a.y), wx), a1 = lerp2h(as_h2(a.z), as_h2(a.w), wx);
+    h2v b0 = lerp2h(as_h2(b.x), as_h2(b.y), wx), b1 = lerp2h(as_h2(b.z), as_h2(b.w), wx);
+    h2v r = lerp2h(lerp2h(a0, a1, wy), lerp2h(b0, b1, wy), wz);
+    return make_float2((float)r.x, (float)r.y);
+  }
+  float2 a0 = lerp2(f2(a.x), f2(a.y), c.ax), a1 = lerp2(f2(a.z), f2(a.w), c.ax);
+  float2 b0 = lerp2(f2(b.x), f2(b.y), c.ax), b1 = lerp2(f2(b.z), f2(b.w), c.ax);
+  return lerp2(lerp2(a0, a1, c.ay), lerp2(b0, b1, c.ay), c.az);
+}
+
+// HF: single channel, the full 2x2x2 footprint in one entry
+template <bool F16>
+__device__ __forceinline__ float sample_hf(const uint4* __restrict__ q, int sh, f3 s) {
+  const float N = (float)(1 << sh);
+  const int m = (1 << sh) - 1;
+  float ux = s.x * N - 0.5f, uy = s.y * N - 0.5f, uz = s.z * N - 0.5f;
+  float fx = floorf(ux), fy = floorf(uy), fz = floorf(uz);
+  float ax = ux - fx, ay = uy - fy, az = uz - fz;
+  int i = (((((int)fz & m) << sh) + ((int)fy & m)) << sh) + ((int)fx & m);
+  uint4 e = q[i];  // (x0y0z0,x1y0z0) (x0y1z0,x1y1z0) (x0y0z1,x1y0z1) (x0y1z1,x1y1z1)
+  float2 p0 = f2(e.x), p1 = f2(e.y), p2 = f2(e.z), p3 = f2(e.w);
+  float c0 = __fmaf_rn(ax, p0.y - p0.x, p0.x), c1 = __fmaf_rn(ax, p1.y - p1.x, p1.x);
+  float c2 = __fmaf_rn(ax, p2.y - p2.x, p2.x), c3 = __fmaf_rn(ax, p3.y - p3.x, p3.x);
+  float d0 = __fmaf_rn(ay, c1 - c0, c0), d1 = __fmaf_rn(ay, c3 - c2, c2);
+  return __fmaf_rn(az, d1 - d0, d0);
+}
+
+// 2D maps: two channels, the 2x2 footprint in one entry
+template <bool F16>
+__device__ __forceinline__ float2 sample_map(const uint4* __restrict__ q, int sh, float sx, float sy) {
+  const float N = (float)(1 << sh);
+  const int m = (1 << sh) - 1;
+  float ux = sx * N - 0.5f, uy = sy * N - 0.5f;
+  float fx = floorf(ux), fy = floorf(uy);
+  float ax = ux - fx, ay = uy - fy;
+  uint4 e = q[((((int)fy & m) << sh) + ((int)fx & m))];
+  if (F16) {
+    h2v wx = {(_Float16)ax, (_Float16)ax}, wy = {(_Float16)ay, (_Float16)ay};
+    h2v r = lerp2h(lerp2h(as_h2(e.x), as_h2(e.y), wx), lerp2h(as_h2(e.z), as_h2(e.w), wx), wy);
+    return make_float2((float)r.x, (float)r.y);
+  }
+  return lerp2(lerp2(f2(e.x), f2(e.y), ax), lerp2(f2(e.z), f2(e.w), ax), ay);
+}
+
+__device__ __forceinline__ float fast_pow(float x, float y) { return __builtin_amdgcn_exp2f(y * __builtin_amdgcn_logf(x)); }
+__device__ __forceinline__ float fast_exp(float x) { return __builtin_amdgcn_exp2f(x * 1.4426950408889634f); }
+__device__ __forceinline__ float rcp(float x) { return __builtin_amdgcn_rcpf(x); }
+
+// Density of one LF sample (sampleLowFrequency, :499-540).  In weather
+// mode the height gradient's two per-step profiles (sc, st) are passed in.
+template <int COV>
+__device__ __forceinline__ float density(float2 lf, float2 w, float cov_const, float sc, float st) {
+  float fbm = clampf(lf.y, 0.0f, 1.0f);
+  // remapClamped(r, fbm - 0.9, 1, 0, 1)
+  float base = clampf((lf.x - (fbm - 0.9f)) * rcp(1.9f - fbm), 0.0f, 1.0f);
+  float cov = cov_const;
+  if (COV == MTC_COVERAGE_WEATHER) {
+    float type = w.y;
+    float a = mixf(st, sc, clampf(type * 2.0f, 0.0f, 1.0f));
+    float b = mixf(sc, st, clampf((type - 0.5f) * 2.0f, 0.0f, 1.0f));
+    base = base * (mixf(a, b, type) * 0.5f);
+    cov = w.x;
+  }
+  // remapClampedBeforeAndAfter(base, cov, 1, 0, 1) * cov
+  float v = clampf(base, cov, 1.0f);
+  return clampf((v - cov) * rcp(1.0f - cov), 0.0f, 1.0f) * cov;
+}
+
+struct FastViews {
+  const uint4* lf;
+  const uint4* hf;
+  const uint4* curl;
+  const uint4* weather;
+  int lf_sh, hf_sh, curl_sh, weather_sh;
+};
+
+// Shell entry/exit of a ray (raySphereIntersection, :229-273, with the
+// t = length(point_world - rO_normalised) quirk of :269).
+template <bool FP64SHELL>
+__device__ __forceinline__ void shell_span(const FrameConsts& fc, const Ray& ray, f3 c, float& start_t, float& end_t,
+                                           f3& startPos) {
+  if (FP64SHELL) {
+    // FP64: C = dot(rO, rO) - 1 cancels in FP32 at |rO| ~ 0.9998
+    // (radius ~6.4e6 m)
+    double ox = ray.o.x, oy = ray.o.y, oz = ray.o.z;
+    double dx = ray.d.x, dy = ray.d.y, dz = ray.d.z;
+    double cx = c.x, cy = -fc.earth_r_d, cz = c.z;
+    double ts[2] = {0.0, 0.0};
+    double sp[3] = {0.0, 0.0, 0.0};
+#pragma unroll
+    for (int k = 0; k < 2; ++k) {
+      double R = k == 0 ? fc.r_in_d : fc.r_out_d;
+      double rx0 = (ox - cx) / R, ry0 = (oy - cy) / R, rz0 = (oz - cz) / R;
+      double A = dx * dx + dy * dy + dz * dz;
+      double B = 2.0 * (dx * rx0 + dy * ry0 + dz * rz0);
+      double C = (rx0 * rx0 + ry0 * ry0 + rz0 * rz0) - 1.0;
+      double disc = B * B - 4.0 * A * C;
+      double t = -1.0;
+      if (disc >= 0.0) {
+        double sq = sqrt(disc);
+        t = (-B - sq) / (2.0 * A);
+        if (t < 0.0) t = (-B + sq) / (2.0 * A);
+      }
+      if (t >= 0.0) {
+        double pxw = (rx0 + t * dx) * R + cx, pyw = (ry0 + t * dy) * R + cy, pzw = (rz0 + t * dz) * R + cz;
+        double ex = pxw - rx0, ey = pyw - ry0, ez = pzw - rz0;
+        ts[k] = sqrt(ex * ex + ey * ey + ez * ez);
+        if (k == 0) {
+          sp[0] = pxw;
+          sp[1] = pyw;
+          sp[2] = pzw;
+        }
+      }
+    }
+    start_t = (float)ts[0];
+    end_t = (float)ts[1];
+    startPos = mk((float)sp[0], (float)sp[1], (float)sp[2]);
+  } else {
+    Isect in = ray_sphere(ray.o, ray.d, c, fc.r_in);
+    Isect out = ray_sphere(ray.o, ray.d, c, fc.r_out);
+    start_t = in.t;
+    end_t = out.t;
+    startPos = in.point;
+  }
+}
+
+// Everything a step needs that is constant along the ray.
+struct RayMarch {
+  Ray ray;
+  f3 c, inner, wk, woff;
+  float start_t, end_t, stepSize, cos_angle, HG_light, inv_thick, inv_div, ls, ws, cone_scale;
+  int pixelID, cap;
+};
+
+// Result of one primary step, independent of the accumulated density.
+struct StepOut {
+  bool valid;   // t_k < end_t and k < cap
+  bool cloud;   // base density > 0
+  float da;     // accumulated-density increment (hfd * accum_scale)
+  float E;      // GetLightEnergy of the step
+  int cone_er;  // cone samples with density > 0
+};
+
+template <int COV, bool F16>
+__device__ __forceinline__ StepOut eval_step(const FrameConsts& fc, const FastViews& fv, const float* sjit,
+                                             const RayMarch& rm, int k) {
+  StepOut o;
+  o.cloud = false;
+  o.da = 0.0f;
+  o.E = 0.0f;
+  o.cone_er = 0;
+  const float t = __fmaf_rn((float)k, rm.stepSize, rm.start_t);
+  o.valid = (t < rm.end_t) && (k < rm.cap);
+  if (!o.valid) return o;
+  const Ray& ray = rm.ray;
+  const int ji = ((rm.pixelID + (int)t) & 15) * 3;
+  const f3 dj = mk(ray.d.x + sjit[ji], ray.d.y + sjit[ji + 1], ray.d.z + sjit[ji + 2]);
+  const f3 pos = ray.o + t * dj;
+  const f3 rel = (pos - rm.inner) * rm.inv_thick;
+  const f3 sp = rel * rm.inv_div;
+  const f3 pe = pos - ray.o;
+  const f3 pc = pos - rm.c;
+  const float lr = __builtin_sqrtf(dot(pe, pe));
+  const float cosT = dot(ray.d, pc) * __builtin_amdgcn_rsqf(dot(pc, pc));
+  const float h = fabsf(cosT * (lr - rm.ls)) * rm.inv_thick;
+  const f3 sk = sp + h * rm.wk + rm.woff;
+  // height-gradient profiles of this step (getDensityHeightGradientForPoint :479-481)
+  float sc = 0.0f, st = 0.0f;
+  if (COV == MTC_COVERAGE_WEATHER) {
+    float hc = clampf(h, 0.0f, 1.0f);
+    sc = fmaxf(0.0f, (hc * 4.0f) * ((hc - 0.65f) * (1.0f / -0.35f)));
+    st = fmaxf(0.0f, (hc * 10.0f) * ((hc - 0.3f) * (1.0f / -0.1f)));
+  }
+  float2 lf = sample_lf<F16>(fv.lf, fv.lf_sh, sk);
+  float2 w = make_float2(0.0f, 0.0f);
+  if (COV == MTC_COVERAGE_WEATHER) w = sample_map<F16>(fv.weather, fv.weather_sh, sp.x * rm.ws, sp.z * rm.ws);
+  const float base = density<COV>(lf, w, fc.coverage_const, sc, st) * fc.base_density_factor;
+  if (base > 0.0f) {
+    o.cloud = true;
+    // erodeCloudWithHighFrequency at (sk, h): one curl + HF fetch serves the
+    // primary erosion and all six cone erosions (:645, :665)
+    float2 cn = sample_map<F16>(fv.curl, fv.curl_sh, sk.x, sk.y);
+    float kk = (1.0f - h) * 0.5f;
+    f3 p2 = mk(sk.x + cn.x * kk, sk.y + cn.y * kk, sk.z);
+    float fh = sample_hf<F16>(fv.hf, fv.hf_sh, p2);
+    float mod = clampf(mixf(fh, 1.0f - fh, clampf(h * 2.0f, 0.0f, 1.0f)), 0.0f, 1.0f);
+    float m = mod * 0.005f;
+    float inv1m = rcp(1.0f - m);
+    o.da = ((base * fc.erosion_scale - m) * inv1m) * fc.accum_scale;
+    float dal = 0.0f;
+#pragma unroll
+    for (int i = 0; i < 6; ++i) {
+      if (i < fc.light_samples) {
+        float s = (float)i * rm.cone_scale;
+        f3 slp = mk(rel.x + fc.kern[i][0] * s, rel.y + fc.kern[i][1] * s, rel.z + fc.kern[i][2] * s);
+        float2 clf = sample_lf<F16>(fv.lf, fv.lf_sh, slp);
+        float2 cw = make_float2(0.0f, 0.0f);
+        if (COV == MTC_COVERAGE_WEATHER) cw = sample_map<F16>(fv.weather, fv.weather_sh, slp.x * rm.ws, slp.z * rm.ws);
+        float cb = density<COV>(clf, cw, fc.coverage_const, sc, st);
+        if (cb > 0.0f) {
+          o.cone_er++;
+          dal += (fc.cone_erosion_scale * cb - m) * inv1m;
+        }
+      }
+    }
+    // GetLightEnergy (:331-388): exp(-dl) once for both identical terms
+    float prim = fast_exp(-dal);
+    float att = fmaxf(remap(rm.cos_angle, 0.7f, 1.0f, prim, prim * 0.25f), prim);
+    float depth = 0.05f + fast_pow(base, clampf(remap(h * 0.125f, 0.3f, 0.85f, 0.5f, 2.0f), 0.5f, 2.0f));
+    float vert = fast_pow(clampf(remap(h * 1.5f, 0.07f, 0.34f, 0.1f, 1.0f), 0.1f, 1.0f), 0.8f);
+    o.E = att * prim * (depth * vert) * rm.HG_light * fc.brightness;
+  }
+  return o;
+}
+
+// G lanes march one ray: lane j of the group evaluates step k0 + j of each
+// chunk of G steps; the group then replays the accumulation / early-exit
+// recurrence of :674-684 over the chunk in step order (identical values in
+// all G lanes), so the result equals the sequential march.  A wave holds
+// 64 / G rays; a 256-lane workgroup covers a tile of 8 x (32 / G) rays.
+template <int COV, bool F16, bool FP64SHELL, bool COUNT, int G>
+__global__ __launch_bounds__(256) void march_kernel(FrameConsts fc, FastViews fv, OutputViews ov, int full) {
+  __shared__ float sjit[48];
+  if (threadIdx.x < 48) sjit[threadIdx.x] = (&fc.step_jitter[0][0])[threadIdx.x];
+  __syncthreads();
+
+  constexpr int RPB = 256 / G;  // rays per workgroup
+  constexpr int TX = 8, TY = RPB / 8;
+  const int lane = threadIdx.x & 63;
+  const int slot = threadIdx.x / G;   // ray slot in the workgroup
+  const int j = threadIdx.x % G;      // step offset within a chunk
+  const int gbase = lane - j;         // first lane of this ray's group (within the wave)
+  const int rx = blockIdx.x * TX + (slot % TX);
+  const int ry = blockIdx.y * TY + (slot / TX);
+  const int W = fc.width, H = fc.height;
+  const int pixelID = fc.frame;
+  const int px = full ? rx : rx * 4 + pixelID / 4;
+  const int py = full ? ry : ry * 4 + pixelID % 4;
+  const bool active = px < W && py < H;
+  unsigned long long n_steps = 0, n_cloud = 0, n_cone = 0, n_cone_er = 0, n_early = 0;
+  unsigned long long n_ocean = 0, n_sky = 0, n_marched = 0;
+  if (active) {
+    float ux = (float)px / (float)W, uy = 1.0f - (float)py / (float)H;
+    Ray ray = cast_ray(fc, fc.view, ux, uy, fc.jit_march[0], fc.jit_march[1]);
+    const float d = ray.d.y;
+    float4 color, mask = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+    if (d < 0.0f) {
+      float a = -ray.d.y * 5.5f;
+      color = make_float4(mixf(0.0f * 0.4f, 0.0f * 0.5f, a), mixf(0.16f * 0.4f, 0.73f * 0.5f, a),
+                          mixf(0.51f * 0.4f, 0.95f * 0.5f, a), 1.0f);
+      n_ocean = 1;
+    } else {
+      f3 bg = atmosphere(fc, ray.d) * fmaxf(fc.bg_min, d);
+      if (d < fc.fade_start) {
+        color = make_float4(bg.x, bg.y, bg.z, 1.0f);
+        n_sky = 1;
+      } else {
+        n_marched = 1;
+        RayMarch rm;
+        rm.ray = ray;
+        rm.c = mk(ray.o.x, -fc.earth_r, ray.o.z);
+        f3 startPos;
+        shell_span<FP64SHELL>(fc, ray, rm.c, rm.start_t, rm.end_t, startPos);
+        float maxSteps = (fc.step_mode == MTC_STEPS_FIXED)
+                             ? (float)fc.steps_max
+                             : floorf(mixf((float)fc.steps_min, (float)fc.steps_max, 1.0f - d));
+        rm.stepSize = (rm.end_t - rm.start_t) / maxSteps;
+        rm.cos_angle = dot(normalize(ray.d), ld3(fc.light_dir));
+        rm.HG_light = fmaxf(hg(rm.cos_angle, fc.hg_g), fc.hg_si * hg(rm.cos_angle, fc.hg_g2));
+        rm.inv_thick = 1.0f / fc.thickness;
+        rm.inv_div = 1.0f / fc.sample_divisor;
+        rm.ls = length(startPos - ray.o);
+        rm.inner = mk(rm.c.x, fc.inner_off, rm.c.z);
+        const float wsk = fc.top_offset * fc.wind_skew;
+        rm.wk = mk(fc.wind_dir[0] * wsk, fc.wind_dir[1] * wsk, fc.wind_dir[2] * wsk);
+        rm.woff = ld3(fc.wind_offset);
+        rm.ws = fc.weather_scale;
+        rm.cone_scale = rm.stepSize * rm.inv_thick;
+        rm.pixelID = pixelID;
+        rm.cap = 4 * max(fc.steps_min, fc.steps_max) + 64;
+
+        float accum = 0.0f, transmittance = 1.0f, col = 0.0f;
+        bool exited = false;
+        for (int k0 = 0;; k0 += G) {
+          StepOut so = eval_step<COV, F16>(fc, fv, sjit, rm, k0 + j);
+          bool last_valid = false;
+#pragma unroll
+          for (int i = 0; i < G; ++i) {
+            const int src = gbase + i;
+            const bool v = G == 1 ? so.valid : (bool)__shfl((int)so.valid, src, 64);
+            const bool cl = G == 1 ? so.cloud : (bool)__shfl((int)so.cloud, src, 64);
+            const float da = G == 1 ? so.da : __shfl(so.da, src, 64);
+            const float E = G == 1 ? so.E : __shfl(so.E, src, 64);
+            const int ce = COUNT ? (G == 1 ? so.cone_er : __shfl(so.cone_er, src, 64)) : 0;
+            last_valid = v;
+            if (v && !exited) {
+              if (COUNT && j == 0) n_steps++;
+              if (cl) {
+                if (COUNT && j == 0) {
+                  n_cloud++;
+                  n_cone += (unsigned)fc.light_samples;
+                  n_cone_er += (unsigned)ce;
+                }
+                accum += da;
+                transmittance = mixf(transmittance, E, 1.0f - accum);
+                col += transmittance;
+              }
+              if (accum >= 1.0f) {
+                accum = 1.0f;
+                exited = true;
+                if (COUNT && j == 0) n_early++;
+              }
+            }
+          }
+          if (exited || !last_valid) break;
+        }
+        float god = accum;
+        accum = accum * smoothstepf(0.0f, 1.0f, fminf(1.0f, remap(ray.d.y, fc.fade_start, fc.fade_end, 0.0f, 1.0f)));
+        color = make_float4(mixf(bg.x, col, accum), mixf(bg.y, col, accum), mixf(bg.z, col, accum), 1.0f);
+        mask = encode_float_rgba(25.0f * fminf(0.05f, 1.0f - god));
+      }
+    }
+    if (j == 0) {
+      const size_t o = (size_t)py * (size_t)W + (size_t)px;
+      ov.color[o] = color;
+      ov.mask[o] = mask;
+    }
+  }
+  if (COUNT) {
+    if (j != 0) n_ocean = n_sky = n_marched = 0;
+    wave_count(ov.counters + 0, n_marched);
+    wave_count(ov.counters + 1, n_ocean);
+    wave_count(ov.counters + 2, n_sky);
+    wave_count(ov.counters + 3, n_steps);
+    wave_count(ov.counters + 4, n_cloud);
+    wave_count(ov.counters + 5, n_cone);
+    wave_count(ov.counters + 6, n_cone_er);
+    wave_count(ov.counters + 7, n_early);
+  }
+}
+
+// ---------------------------------------------------------------------------
+// footprint packing (run once per volume upload)
+// ---------------------------------------------------------------------------
+__device__ __forceinline__ void texel4(const void* src, int fmt, size_t idx, float v[4]) {
+  if (fmt == MTC_FMT_FP16) {
+    uint2 t = ((const uint2*)src)[idx];
+    float2 a = f2(t.x), b = f2(t.y);
+    v[0] = a.x;
+    v[1] = a.y;
+    v[2] = b.x;
+    v[3] = b.y;
+  } else {
+    uint32_t t = ((const uint32_t*)src)[idx];
+    const float k = MTC_INV255;
+    v[0] = (float)(t & 0xffu) * k;
+    v[1] = (float)((t >> 8) & 0xffu) * k;
+    v[2] = (float)((t >> 16) & 0xffu) * k;
+    v[3] = (float)(t >> 24) * k;
+  }
+}
+
+// LF -> (r, .625g + .25b + .125a) at the 2x2 (x, y) footprint of each texel
+__global__ __launch_bounds__(256) void pack_lf_kernel(const void* src, int fmt, int sh, uint4* out) {
+  const int N = 1 << sh, m = N - 1;
+  const size_t idx = (size_t)blockIdx.x * 256 + threadIdx.x;
+  if (idx >= (size_t)N * N * N) return;
+  int x = (int)(idx & m), y = (int)((idx >> sh) & m), z = (int)(idx >> (2 * sh));
+  uint32_t e[4];
+  for (int k = 0; k < 4; ++k) {
+    int xx = (x + (k & 1)) & m, yy = (y + (k >> 1)) & m;
+    float v[4];
+    texel4(src, fmt, (((size_t)z << sh) + yy) * N + xx, v);
+    e[k] = pack_h2(v[0], (v[1] * 0.625f + v[2] * 0.25f) + v[3] * 0.125f);
+  }
+  out[idx] = make_uint4(e[0], e[1], e[2], e[3]);
+}
+
+// HF -> .625r + .25g + .125b at the 2x2x2 footprint
+__global__ __launch_bounds__(256) void pack_hf_kernel(const void* src, int fmt, int sh, uint4* out) {
+  const int N = 1 << sh, m = N - 1;
+  const size_t idx = (size_t)blockIdx.x * 256 + threadIdx.x;
+  if (idx >= (size_t)N * N * N) return;
+  int x = (int)(idx & m), y = (int)((idx >> sh) & m), z = (int)(idx >> (2 * sh));
+  float f[8];
+  for (int k = 0; k < 8; ++k) {
+    int xx = (x + (k & 1)) & m, yy = (y + ((k >> 1) & 1)) & m, zz = (z + (k >> 2)) & m;
+    float v[4];
+    texel4(src, fmt, (((size_t)zz << sh) + yy) * N + xx, v);
+    f[k] = (v[0] * 0.625f + v[1] * 0.25f) + v[2] * 0.125f;
+  }
+  out[idx] = make_uint4(pack_h2(f[0], f[1]), pack_h2(f[2], f[3]), pack_h2(f[4], f[5]), pack_h2(f[6], f[7]));
+}
+
+// 2D RGBA8 map -> (r, g) at the 2x2 footprint
+__global__ __launch_bounds__(256) void pack_map_kernel(const uint32_t* src, int sh, uint4* out) {
+  const int N = 1 << sh, m = N - 1;
+  const int idx = blockIdx.x * 256 + threadIdx.x;
+  if (idx >= N * N) return;
+  int x = idx & m, y = idx >> sh;
+  uint32_t e[4];
+  for (int k = 0; k < 4; ++k) {
+    int xx = (x + (k & 1)) & m, yy = (y + (k >> 1)) & m;
+    float v[4];
+    texel4(src, MTC_FMT_UNORM8, ((size_t)yy << sh) + xx, v);
+    e[k] = pack_h2(v[0], v[1]);
+  }
+  out[idx] = make_uint4(e[0], e[1], e[2], e[3]);
+}
+
+}  // namespace fast
+}  // namespace mtc
+
+hipError_t pack_fast_resources(const VolumeViews& vv, uint4* lf, uint4* hf, uint4* curl, uint4* weather,
+                               hipStream_t s) {
+  size_t n = (size_t)vv.lf_dim * vv.lf_dim * vv.lf_dim;
+  hipLaunchKernelGGL(mtc::fast::pack_lf_kernel, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, vv.lf, vv.format,
+                     vv.lf_shift, lf);
+  n = (size_t)vv.hf_dim * vv.hf_dim * vv.hf_dim;
+  hipLaunchKernelGGL(mtc::fast::pack_hf_kernel, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, vv.hf, vv.format,
+                     vv.hf_shift, hf);
+  n = (size_t)vv.curl_dim * vv.curl_dim;
+  hipLaunchKernelGGL(mtc::fast::pack_map_kernel, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, vv.curl,
+                     vv.curl_shift, curl);
+  n = (size_t)vv.weather_dim * vv.weather_dim;
+  hipLaunchKernelGGL(mtc::fast::pack_map_kernel, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, vv.weather,
+                     vv.weather_shift, weather);
+  return hipGetLastError();
+}
+
+template <int COV, bool F16, bool FP64, int G>
+static void launch_v(hipStream_t s, const FrameConsts& fc, const mtc::fast::FastViews& fv, const OutputViews& ov,
+                     int full) {
+  int rays_x = full ? fc.width : (fc.width + 3) / 4;
+  int rays_y = full ? fc.height : (fc.height + 3) / 4;
+  constexpr int TX = 8, TY = 256 / G / 8;
+  dim3 grid((rays_x + TX - 1) / TX, (rays_y + TY - 1) / TY), block(256);
+  if (ov.counters) hipLaunchKernelGGL((mtc::fast::march_kernel<COV, F16, FP64, true, G>), grid, block, 0, s, fc, fv, ov, full);
+  else hipLaunchKernelGGL((mtc::fast::march_kernel<COV, F16, FP64, false, G>), grid, block, 0, s, fc, fv, ov, full);
+}
+
+template <bool F16, bool FP64, int G>
+static void launch_cov(hipStream_t s, const FrameConsts& fc, const mtc::fast::FastViews& fv, const OutputViews& ov,
+                       int full) {
+  if (fc.coverage_mode == MTC_COVERAGE_WEATHER) launch_v<MTC_COVERAGE_WEATHER, F16, FP64, G>(s, fc, fv, ov, full);
+  else launch_v<MTC_COVERAGE_CONSTANT, F16, FP64, G>(s, fc, fv, ov, full);
+}
+
+template <bool F16, bool FP64>
+static bool launch_g(int lanes_per_ray, hipStream_t s, const FrameConsts& fc, const mtc::fast::FastViews& fv,
+                     const OutputViews& ov, int full) {
+  switch (lanes_per_ray) {
+    case 1: launch_cov<F16, FP64, 1>(s, fc, fv, ov, full); return true;
+    case 2: launch_cov<F16, FP64, 2>(s, fc, fv, ov, full); return true;
+    case 4: launch_cov<F16, FP64, 4>(s, fc, fv, ov, full); return true;
+    case 8: launch_cov<F16, FP64, 8>(s, fc, fv, ov, full); return true;
+    default: return false;
+  }
+}
+
+// variant 0 = fast (FP32 filtering), 1 = precision (packed FP16
+// filtering); lanes_per_ray in {1, 2, 4, 8}
+hipError_t launch_march_fast(const FrameConsts& fc, const uint4* lf, const uint4* hf, const uint4* curl,
+                             const uint4* weather, const VolumeViews& vv, const OutputViews& ov, int full, int variant,
+                             int lanes_per_ray, hipStream_t stream) {
+  mtc::fast::FastViews fv{lf, hf, curl, weather, vv.lf_shift, vv.hf_shift, vv.curl_shift, vv.weather_shift};
+  // Both variants keep the shell quadratic in FP64: in FP32 its hit points
+  // move by 0.3-1 m (C = dot(rO, rO) - 1 cancels), which shifts every t_k and
+  // flips the int(t) jitter index of most steps (:631).
+  bool ok = variant == 0 ? launch_g<false, true>(lanes_per_ray, stream, fc, fv, ov, full)
+                         : launch_g<true, true>(lanes_per_ray, stream, fc, fv, ov, full);
+  if (!ok) return hipErrorInvalidValue;
+  return hipGetLastError();
+}

@@ -16,14 +16,21 @@
 
 #include <cstdio>
 #include <cstring>
+#include <array>
 #include <new>
 #include <string>
+#include <vector>
 
 #include "meteoros_cloud.h"
 #include "runtime_internal.h"
 
 hipError_t launch_march_plain(const FrameConsts& fc, const VolumeViews& vv, const OutputViews& ov, int full,
                               hipStream_t stream);
+hipError_t launch_march_fast(const FrameConsts& fc, const uint4* lf, const uint4* hf, const uint4* curl,
+                             const uint4* weather, const VolumeViews& vv, const OutputViews& ov, int full, int variant,
+                             int lanes_per_ray, hipStream_t stream);
+hipError_t pack_fast_resources(const VolumeViews& vv, uint4* lf, uint4* hf, uint4* curl, uint4* weather,
+                               hipStream_t s);
 hipError_t launch_reproject(const FrameConsts& fc, const OutputViews& ov, bool indices, hipStream_t stream);
 hipError_t launch_noise(int kind, void* out, int N, int fmt, uint32_t seed, hipStream_t stream);
 
@@ -62,8 +69,19 @@ struct mtc_ctx {
   float4* img[2] = {nullptr, nullptr};
   float4* mask = nullptr;
   unsigned long long* counters = nullptr;
+  // footprint-packed copies of the resources read by the fast kernels
+  uint4* q_lf = nullptr;
+  uint4* q_hf = nullptr;
+  uint4* q_curl = nullptr;
+  uint4* q_weather = nullptr;
   int latest = 0;
+  int lanes_per_ray = 8;  // fast kernels: lanes cooperating on one ray
   bool volumes_ready = false;
+  // MTC_FLAG_TIMING: event triples (before reprojection, between, after
+  // march) of frames not yet folded into the totals
+  std::vector<std::array<hipEvent_t, 3>> pending;
+  std::vector<hipEvent_t> event_pool;
+  mtc_timings totals = {0, 0.0, 0.0};
   size_t lf_bytes = 0, hf_bytes = 0, curl_bytes = 0, weather_bytes = 0;
 };
 
@@ -71,10 +89,45 @@ namespace {
 
 size_t texel_bytes(int fmt) { return fmt == MTC_FMT_FP16 ? 8 : 4; }
 
+hipEvent_t take_event(mtc_ctx* c) {
+  if (!c->event_pool.empty()) {
+    hipEvent_t e = c->event_pool.back();
+    c->event_pool.pop_back();
+    return e;
+  }
+  hipEvent_t e = nullptr;
+  if (hipEventCreate(&e) != hipSuccess) return nullptr;
+  return e;
+}
+
+// Fold the oldest `keep`-exceeding pending frames into the totals.
+hipError_t fold_timings(mtc_ctx* c, size_t keep) {
+  while (c->pending.size() > keep) {
+    auto ev = c->pending.front();
+    c->pending.erase(c->pending.begin());
+    hipError_t e = hipEventSynchronize(ev[2]);
+    if (e != hipSuccess) return e;
+    float a = 0.0f, b = 0.0f;
+    if (ev[0] && ev[1]) (void)hipEventElapsedTime(&a, ev[0], ev[1]);
+    (void)hipEventElapsedTime(&b, ev[1], ev[2]);
+    c->totals.frames += 1;
+    c->totals.reproject_ms += a;
+    c->totals.march_ms += b;
+    for (hipEvent_t x : ev)
+      if (x) c->event_pool.push_back(x);
+  }
+  return hipSuccess;
+}
+
 void release(mtc_ctx* c) {
   if (!c) return;
   (void)hipSetDevice(c->desc.device);
   if (c->stream) (void)hipStreamSynchronize(c->stream);
+  (void)hipDeviceSynchronize();
+  for (auto& ev : c->pending)
+    for (hipEvent_t x : ev)
+      if (x) (void)hipEventDestroy(x);
+  for (hipEvent_t x : c->event_pool) (void)hipEventDestroy(x);
   (void)hipFree(c->lf);
   (void)hipFree(c->hf);
   (void)hipFree(c->curl);
@@ -83,6 +136,10 @@ void release(mtc_ctx* c) {
   (void)hipFree(c->img[1]);
   (void)hipFree(c->mask);
   (void)hipFree(c->counters);
+  (void)hipFree(c->q_lf);
+  (void)hipFree(c->q_hf);
+  (void)hipFree(c->q_curl);
+  (void)hipFree(c->q_weather);
   if (c->stream) (void)hipStreamDestroy(c->stream);
   delete c;
 }
@@ -119,6 +176,12 @@ int launch_march(mtc_ctx* c, const FrameConsts& fc, const OutputViews& ov, int f
   hipError_t e;
   switch (c->desc.kernel) {
     case MTC_KERNEL_PLAIN: e = launch_march_plain(fc, vv, ov, full, s); break;
+    case MTC_KERNEL_FAST:
+      e = launch_march_fast(fc, c->q_lf, c->q_hf, c->q_curl, c->q_weather, vv, ov, full, 0, c->lanes_per_ray, s);
+      break;
+    case MTC_KERNEL_PRECISION:
+      e = launch_march_fast(fc, c->q_lf, c->q_hf, c->q_curl, c->q_weather, vv, ov, full, 1, c->lanes_per_ray, s);
+      break;
     default: return mtc_fail(MTC_ERR_UNSUPPORTED, "march: kernel variant not available in this build");
   }
   HIP_TRY(e, "march kernel launch");
@@ -156,7 +219,7 @@ int mtc_create(const mtc_desc* d, mtc_ctx** out) {
   if (d->kernel < MTC_KERNEL_PLAIN || d->kernel > MTC_KERNEL_LEARNED)
     return mtc_fail(MTC_ERR_INVALID_ARG, "mtc_create: unknown kernel variant");
   if (d->kernel == MTC_KERNEL_LEARNED) return mtc_fail(MTC_ERR_UNSUPPORTED, "mtc_create: learned-light kernel not built");
-  if (d->flags & ~(MTC_FLAG_COUNTERS | MTC_FLAG_FULL_MARCH)) return mtc_fail(MTC_ERR_INVALID_ARG, "mtc_create: unknown flags");
+  if (d->flags & ~(MTC_FLAG_COUNTERS | MTC_FLAG_FULL_MARCH | MTC_FLAG_TIMING)) return mtc_fail(MTC_ERR_INVALID_ARG, "mtc_create: unknown flags");
   int ndev = 0;
   if (hipGetDeviceCount(&ndev) != hipSuccess || ndev <= 0)
     return mtc_fail(MTC_ERR_HIP, "mtc_create: no HIP device available");
@@ -180,6 +243,13 @@ int mtc_create(const mtc_desc* d, mtc_ctx** out) {
   if (e == hipSuccess) e = hipMalloc((void**)&c->img[1], px * sizeof(float4));
   if (e == hipSuccess) e = hipMalloc((void**)&c->mask, px * sizeof(float4));
   if (e == hipSuccess) e = hipMalloc((void**)&c->counters, 16 * sizeof(unsigned long long));
+  if (d->kernel != MTC_KERNEL_PLAIN) {
+    const size_t lf3 = (size_t)v.lf_dim * v.lf_dim * v.lf_dim, hf3 = (size_t)v.hf_dim * v.hf_dim * v.hf_dim;
+    if (e == hipSuccess) e = hipMalloc((void**)&c->q_lf, lf3 * sizeof(uint4));
+    if (e == hipSuccess) e = hipMalloc((void**)&c->q_hf, hf3 * sizeof(uint4));
+    if (e == hipSuccess) e = hipMalloc((void**)&c->q_curl, (size_t)v.curl_dim * v.curl_dim * sizeof(uint4));
+    if (e == hipSuccess) e = hipMalloc((void**)&c->q_weather, (size_t)v.weather_dim * v.weather_dim * sizeof(uint4));
+  }
   if (e == hipSuccess) e = hipMemsetAsync(c->img[0], 0, px * sizeof(float4), c->stream);
   if (e == hipSuccess) e = hipMemsetAsync(c->img[1], 0, px * sizeof(float4), c->stream);
   if (e == hipSuccess) e = hipMemsetAsync(c->mask, 0, px * sizeof(float4), c->stream);
@@ -210,6 +280,10 @@ int mtc_set_volumes(mtc_ctx* c, const void* lf, const void* hf, const void* curl
   HIP_TRY(hipMemcpyAsync(c->curl, curl, c->curl_bytes, hipMemcpyHostToDevice, c->stream), "mtc_set_volumes: curl");
   HIP_TRY(hipMemcpyAsync(c->weather, weather, c->weather_bytes, hipMemcpyHostToDevice, c->stream),
           "mtc_set_volumes: weather");
+  if (c->q_lf) {
+    HIP_TRY(pack_fast_resources(volume_views(c), c->q_lf, c->q_hf, c->q_curl, c->q_weather, c->stream),
+            "mtc_set_volumes: footprint packing");
+  }
   HIP_TRY(hipStreamSynchronize(c->stream), "mtc_set_volumes: sync");
   c->volumes_ready = true;
   return MTC_OK;
@@ -224,6 +298,10 @@ int mtc_generate_volumes(mtc_ctx* c, const mtc_noise_desc* nd) {
   HIP_TRY(launch_noise(1, c->hf, v.hf_dim, v.format, nd->seed_volumes, c->stream), "noise hf");
   HIP_TRY(launch_noise(2, c->curl, v.curl_dim, MTC_FMT_UNORM8, nd->seed_volumes, c->stream), "noise curl");
   HIP_TRY(launch_noise(3, c->weather, v.weather_dim, MTC_FMT_UNORM8, nd->seed_weather, c->stream), "noise weather");
+  if (c->q_lf) {
+    HIP_TRY(pack_fast_resources(volume_views(c), c->q_lf, c->q_hf, c->q_curl, c->q_weather, c->stream),
+            "mtc_generate_volumes: footprint packing");
+  }
   HIP_TRY(hipStreamSynchronize(c->stream), "mtc_generate_volumes: sync");
   c->volumes_ready = true;
   return MTC_OK;
@@ -259,9 +337,26 @@ int mtc_frame(mtc_ctx* c, const mtc_camera* cur, const mtc_camera* old, const mt
   ov.counters = (c->desc.flags & MTC_FLAG_COUNTERS) ? c->counters : nullptr;
   ov.reproj_idx = nullptr;
   const int full = (c->desc.flags & MTC_FLAG_FULL_MARCH) ? 1 : 0;
-  if (!full) HIP_TRY(launch_reproject(fc, ov, false, s), "reprojection kernel launch");
+  const bool timing = (c->desc.flags & MTC_FLAG_TIMING) != 0;
+  std::array<hipEvent_t, 3> ev = {nullptr, nullptr, nullptr};
+  if (timing) {
+    HIP_TRY(fold_timings(c, 255), "mtc_frame: timing events");
+    for (int k = full ? 1 : 0; k < 3; ++k) {
+      ev[k] = take_event(c);
+      if (!ev[k]) return mtc_fail(MTC_ERR_HIP, "mtc_frame: hipEventCreate failed");
+    }
+  }
+  if (!full) {
+    if (timing) HIP_TRY(hipEventRecord(ev[0], s), "mtc_frame: event");
+    HIP_TRY(launch_reproject(fc, ov, false, s), "reprojection kernel launch");
+  }
+  if (timing) HIP_TRY(hipEventRecord(ev[1], s), "mtc_frame: event");
   rc = launch_march(c, fc, ov, full, s);
   if (rc) return rc;
+  if (timing) {
+    HIP_TRY(hipEventRecord(ev[2], s), "mtc_frame: event");
+    c->pending.push_back(ev);
+  }
   c->latest = dst;
   return MTC_OK;
 }
@@ -357,6 +452,24 @@ int mtc_reset_counters(mtc_ctx* c) {
   if (rc) return rc;
   HIP_TRY(hipMemsetAsync(c->counters, 0, 16 * sizeof(unsigned long long), c->stream), "mtc_reset_counters");
   HIP_TRY(hipStreamSynchronize(c->stream), "mtc_reset_counters: sync");
+  return MTC_OK;
+}
+
+int mtc_get_timings(mtc_ctx* c, mtc_timings* out) {
+  int rc = check_ctx(c, "mtc_get_timings");
+  if (rc) return rc;
+  if (!out) return mtc_fail(MTC_ERR_INVALID_ARG, "mtc_get_timings: null pointer");
+  if (!(c->desc.flags & MTC_FLAG_TIMING)) return mtc_fail(MTC_ERR_STATE, "mtc_get_timings: context built without MTC_FLAG_TIMING");
+  HIP_TRY(fold_timings(c, 0), "mtc_get_timings");
+  *out = c->totals;
+  return MTC_OK;
+}
+
+int mtc_reset_timings(mtc_ctx* c) {
+  int rc = check_ctx(c, "mtc_reset_timings");
+  if (rc) return rc;
+  HIP_TRY(fold_timings(c, 0), "mtc_reset_timings");
+  c->totals = mtc_timings{0, 0.0, 0.0};
   return MTC_OK;
 }
 

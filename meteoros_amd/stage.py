@@ -194,6 +194,14 @@ class CloudStage:
     def reset_counters(self) -> None:
         check(lib().mtc_reset_counters(self._ctx), "mtc_reset_counters")
 
+    def timings(self) -> dict:
+        t = A.MtcTimings()
+        check(lib().mtc_get_timings(self._ctx, C.byref(t)), "mtc_get_timings")
+        return {"frames": int(t.frames), "reproject_ms": float(t.reproject_ms), "march_ms": float(t.march_ms)}
+
+    def reset_timings(self) -> None:
+        check(lib().mtc_reset_timings(self._ctx), "mtc_reset_timings")
+
     def reprojection_indices(self, cam, old, time, params) -> np.ndarray:
         n = self.width * self.height * int(params.motion_blur_samples)
         idx = np.empty(n, dtype=np.int32)
