@@ -229,7 +229,8 @@ typedef struct mtc_desc {
   int32_t kernel;           /* MTC_KERNEL_* */
   int32_t flags;            /* MTC_FLAG_*   */
   int32_t device;           /* HIP device ordinal */
-  int32_t reserved;
+  int32_t lanes_per_ray;    /* fast kernels: lanes cooperating on one ray,
+                               1, 2, 4 or 8 (0 = default 8) */
 } mtc_desc;
 
 /* Per-frame work counters (counting build / MTC_FLAG_COUNTERS). */

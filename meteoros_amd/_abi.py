@@ -86,7 +86,8 @@ class MtcNoiseDesc(C.Structure):
 
 class MtcDesc(C.Structure):
     _fields_ = [("width", C.c_int32), ("height", C.c_int32), ("volumes", MtcVolumeDesc),
-                ("kernel", C.c_int32), ("flags", C.c_int32), ("device", C.c_int32), ("reserved", C.c_int32)]
+                ("kernel", C.c_int32), ("flags", C.c_int32), ("device", C.c_int32),
+                ("lanes_per_ray", C.c_int32)]
 
 
 class MtcCounters(C.Structure):

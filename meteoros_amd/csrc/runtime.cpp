@@ -219,6 +219,9 @@ int mtc_create(const mtc_desc* d, mtc_ctx** out) {
   if (d->kernel < MTC_KERNEL_PLAIN || d->kernel > MTC_KERNEL_LEARNED)
     return mtc_fail(MTC_ERR_INVALID_ARG, "mtc_create: unknown kernel variant");
   if (d->kernel == MTC_KERNEL_LEARNED) return mtc_fail(MTC_ERR_UNSUPPORTED, "mtc_create: learned-light kernel not built");
+  if (d->lanes_per_ray != 0 && d->lanes_per_ray != 1 && d->lanes_per_ray != 2 && d->lanes_per_ray != 4 &&
+      d->lanes_per_ray != 8)
+    return mtc_fail(MTC_ERR_INVALID_ARG, "mtc_create: lanes_per_ray must be 0, 1, 2, 4 or 8");
   if (d->flags & ~(MTC_FLAG_COUNTERS | MTC_FLAG_FULL_MARCH | MTC_FLAG_TIMING)) return mtc_fail(MTC_ERR_INVALID_ARG, "mtc_create: unknown flags");
   int ndev = 0;
   if (hipGetDeviceCount(&ndev) != hipSuccess || ndev <= 0)
@@ -229,6 +232,7 @@ int mtc_create(const mtc_desc* d, mtc_ctx** out) {
   mtc_ctx* c = new (std::nothrow) mtc_ctx;
   if (!c) return mtc_fail(MTC_ERR_OUT_OF_MEMORY, "mtc_create: host allocation failed");
   c->desc = *d;
+  c->lanes_per_ray = d->lanes_per_ray ? d->lanes_per_ray : 8;
   c->lf_bytes = (size_t)v.lf_dim * v.lf_dim * v.lf_dim * texel_bytes(v.format);
   c->hf_bytes = (size_t)v.hf_dim * v.hf_dim * v.hf_dim * texel_bytes(v.format);
   c->curl_bytes = (size_t)v.curl_dim * v.curl_dim * 4;

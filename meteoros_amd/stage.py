@@ -106,13 +106,14 @@ class CloudStage:
     """The compute half of Renderer: resources + Frame() for one context."""
 
     def __init__(self, width: int, height: int, volumes: VolumeSpec | None = None,
-                 kernel: int = A.MTC_KERNEL_PLAIN, flags: int = 0, device: int = 0):
+                 kernel: int = A.MTC_KERNEL_PLAIN, flags: int = 0, device: int = 0, lanes_per_ray: int = 0):
         self.width, self.height = int(width), int(height)
         self.volumes = volumes or VolumeSpec()
         d = A.MtcDesc()
         d.width, d.height = self.width, self.height
         d.volumes = self.volumes.desc()
         d.kernel, d.flags, d.device = int(kernel), int(flags), int(device)
+        d.lanes_per_ray = int(lanes_per_ray)
         self._ctx = C.c_void_p()
         check(lib().mtc_create(C.byref(d), C.byref(self._ctx)), "mtc_create")
 

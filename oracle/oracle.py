@@ -216,3 +216,50 @@ def atmosphere(sun, params, direction, origin=(0.0, 0.0, 0.0), precision=32):
 
 def to_half(f: float) -> int:
     return int(lib().orc_to_half(f))
+
+
+def ray_directions(cam, time, xy, W, H):
+    """castRay (cloudRayMarch.comp:194-226) in float64 numpy for a list of
+    pixels: used to locate the sun disk in an image."""
+    xy = np.asarray(xy).reshape(-1, 2)
+    v = np.array(cam.view, dtype=np.float64).reshape(4, 4)  # v[c][r], column-major
+    right = v[:3, 0] / np.linalg.norm(v[:3, 0])
+    up = v[:3, 1] / np.linalg.norm(v[:3, 1])
+    look = -v[:3, 2] / np.linalg.norm(v[:3, 2])
+    idx = int(time.frame_count_mod16) // 2
+    hs = [time.halton1, time.halton2, time.halton3, time.halton4]
+    jx, jy = (hs[0][idx], hs[1][idx]) if idx < 4 else (hs[2][idx - 4], hs[3][idx - 4])
+    ux = xy[:, 0] / W
+    uy = 1.0 - xy[:, 1] / H
+    nx = ux * 2 - 1 + jx / W
+    ny = uy * 2 - 1 + jy / H
+    d = (look[None] + (nx * cam.tan_fov_by2[0])[:, None] * right[None]
+         + (ny * cam.tan_fov_by2[1])[:, None] * up[None])
+    return d / np.linalg.norm(d, axis=1, keepdims=True)
+
+
+SUN_DISK_RADIUS_DEG = 0.6
+
+
+def outside_sun_disk(cam, time, params, xy, W, H, radius_deg=SUN_DISK_RADIUS_DEG):
+    """True for pixels farther than radius_deg from the background sun.
+
+    The sun-disk term of the Preetham background (cloudRayMarch.comp:460-461)
+    is smoothstep(cos 0.533 deg, +2e-5, cos theta) * 15000 * sunE: over the
+    disk's edge a change of one float ulp in the ray direction moves a
+    pixel's value by ~1% of ~3e4, so those few pixels are not a measure of
+    arithmetic accuracy and the image PSNR leaves them out."""
+    d = ray_directions(cam, time, xy, W, H)
+    s = np.array(params.sky_sun_location[:3], dtype=np.float64)
+    s /= np.linalg.norm(s)
+    return d @ s < np.cos(np.radians(radius_deg))
+
+
+def psnr_vs_fp64(got_rgb, ref_rgb, keep):
+    """PSNR over RGB of the kept pixels, peak = max of the FP64 reference."""
+    g = np.asarray(got_rgb, dtype=np.float64)[keep]
+    r = np.asarray(ref_rgb, dtype=np.float64)[keep]
+    mse = float(np.mean((g - r) ** 2))
+    if mse == 0.0:
+        return float("inf")
+    return 10.0 * np.log10(float(r.max()) ** 2 / mse)
