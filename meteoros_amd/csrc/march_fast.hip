@@ -213,23 +213,30 @@ struct RayMarch {
   int pixelID, cap;
 };
 
-// Result of one primary step, independent of the accumulated density.
-struct StepOut {
-  bool valid;   // t_k < end_t and k < cap
-  bool cloud;   // base density > 0
-  float da;     // accumulated-density increment (hfd * accum_scale)
-  float E;      // GetLightEnergy of the step
-  int cone_er;  // cone samples with density > 0
+// One primary step as seen by its lane, before the cone march.
+struct Prim {
+  bool valid;  // t_k < end_t and k < cap
+  bool cloud;  // base density > 0
+  f3 rel;      // getRelativePositionInAtmosphere(pos) (cone samples start here)
+  float h;     // relative height
+  float base;  // base density * base_density_factor
+  float sc, st;     // height-gradient profiles of the step
+  float m, inv1m;   // erosion: high_freq_modifier * 0.005 and 1 / (1 - m)
+  float da;         // accumulated-density increment of the step
 };
 
 template <int COV, bool F16>
-__device__ __forceinline__ StepOut eval_step(const FrameConsts& fc, const FastViews& fv, const float* sjit,
+__device__ __forceinline__ Prim eval_primary(const FrameConsts& fc, const FastViews& fv, const float* sjit,
                                              const RayMarch& rm, int k) {
-  StepOut o;
+  Prim o;
   o.cloud = false;
   o.da = 0.0f;
-  o.E = 0.0f;
-  o.cone_er = 0;
+  o.m = 0.0f;
+  o.inv1m = 1.0f;
+  o.sc = o.st = 0.0f;
+  o.h = 0.0f;
+  o.base = 0.0f;
+  o.rel = mk(0.0f, 0.0f, 0.0f);
   const float t = __fmaf_rn((float)k, rm.stepSize, rm.start_t);
   o.valid = (t < rm.end_t) && (k < rm.cap);
   if (!o.valid) return o;
@@ -245,9 +252,9 @@ __device__ __forceinline__ StepOut eval_step(const FrameConsts& fc, const FastVi
   const float cosT = dot(ray.d, pc) * __builtin_amdgcn_rsqf(dot(pc, pc));
   const float h = fabsf(cosT * (lr - rm.ls)) * rm.inv_thick;
   const f3 sk = sp + h * rm.wk + rm.woff;
-  // height-gradient profiles of this step (getDensityHeightGradientForPoint :479-481)
   float sc = 0.0f, st = 0.0f;
   if (COV == MTC_COVERAGE_WEATHER) {
+    // getDensityHeightGradientForPoint profiles (:479-481)
     float hc = clampf(h, 0.0f, 1.0f);
     sc = fmaxf(0.0f, (hc * 4.0f) * ((hc - 0.65f) * (1.0f / -0.35f)));
     st = fmaxf(0.0f, (hc * 10.0f) * ((hc - 0.3f) * (1.0f / -0.1f)));
@@ -256,61 +263,59 @@ __device__ __forceinline__ StepOut eval_step(const FrameConsts& fc, const FastVi
   float2 w = make_float2(0.0f, 0.0f);
   if (COV == MTC_COVERAGE_WEATHER) w = sample_map<F16>(fv.weather, fv.weather_sh, sp.x * rm.ws, sp.z * rm.ws);
   const float base = density<COV>(lf, w, fc.coverage_const, sc, st) * fc.base_density_factor;
+  o.rel = rel;
+  o.h = h;
+  o.sc = sc;
+  o.st = st;
+  o.base = base;
   if (base > 0.0f) {
     o.cloud = true;
     // erodeCloudWithHighFrequency at (sk, h): one curl + HF fetch serves the
-    // primary erosion and all six cone erosions (:645, :665)
+    // primary erosion and all cone erosions of the step (:645, :665)
     float2 cn = sample_map<F16>(fv.curl, fv.curl_sh, sk.x, sk.y);
     float kk = (1.0f - h) * 0.5f;
     f3 p2 = mk(sk.x + cn.x * kk, sk.y + cn.y * kk, sk.z);
     float fh = sample_hf<F16>(fv.hf, fv.hf_sh, p2);
     float mod = clampf(mixf(fh, 1.0f - fh, clampf(h * 2.0f, 0.0f, 1.0f)), 0.0f, 1.0f);
-    float m = mod * 0.005f;
-    float inv1m = rcp(1.0f - m);
-    o.da = ((base * fc.erosion_scale - m) * inv1m) * fc.accum_scale;
-    float dal = 0.0f;
-#pragma unroll
-    for (int i = 0; i < 6; ++i) {
-      if (i < fc.light_samples) {
-        float s = (float)i * rm.cone_scale;
-        f3 slp = mk(rel.x + fc.kern[i][0] * s, rel.y + fc.kern[i][1] * s, rel.z + fc.kern[i][2] * s);
-        float2 clf = sample_lf<F16>(fv.lf, fv.lf_sh, slp);
-        float2 cw = make_float2(0.0f, 0.0f);
-        if (COV == MTC_COVERAGE_WEATHER) cw = sample_map<F16>(fv.weather, fv.weather_sh, slp.x * rm.ws, slp.z * rm.ws);
-        float cb = density<COV>(clf, cw, fc.coverage_const, sc, st);
-        if (cb > 0.0f) {
-          o.cone_er++;
-          dal += (fc.cone_erosion_scale * cb - m) * inv1m;
-        }
-      }
-    }
-    // GetLightEnergy (:331-388): exp(-dl) once for both identical terms
-    float prim = fast_exp(-dal);
-    float att = fmaxf(remap(rm.cos_angle, 0.7f, 1.0f, prim, prim * 0.25f), prim);
-    float depth = 0.05f + fast_pow(base, clampf(remap(h * 0.125f, 0.3f, 0.85f, 0.5f, 2.0f), 0.5f, 2.0f));
-    float vert = fast_pow(clampf(remap(h * 1.5f, 0.07f, 0.34f, 0.1f, 1.0f), 0.1f, 1.0f), 0.8f);
-    o.E = att * prim * (depth * vert) * rm.HG_light * fc.brightness;
+    o.m = mod * 0.005f;
+    o.inv1m = rcp(1.0f - o.m);
+    o.da = ((base * fc.erosion_scale - o.m) * o.inv1m) * fc.accum_scale;
   }
   return o;
 }
 
+// Per-wave LDS work area of the compacted cone march.
+struct ConeLds {
+  float4 own[64][2];     // per in-cloud step (by rank): rel.xyz, cone_scale | sc, st, m, inv1m
+  float contrib[6 * 64]; // one erosion result per cone sample
+  unsigned char hit[6 * 64];
+};
+
 // G lanes march one ray: lane j of the group evaluates step k0 + j of each
-// chunk of G steps; the group then replays the accumulation / early-exit
-// recurrence of :674-684 over the chunk in step order (identical values in
-// all G lanes), so the result equals the sequential march.  A wave holds
-// 64 / G rays; a 256-lane workgroup covers a tile of 8 x (32 / G) rays.
+// chunk of G steps.  The cone samples of all in-cloud steps of the wave are
+// then compacted: a ballot ranks the in-cloud lanes, each publishes its
+// step's data to LDS, and all 64 lanes process the (steps x samples) work
+// list in ceil(n * samples / 64) rounds; each owner sums its samples back.
+// Finally the group replays the accumulation / early-exit recurrence of
+// :674-684 over the chunk in step order (identical values in all G lanes),
+// so the result equals the sequential march.  The loop is wave-uniform: a
+// lane whose ray has finished still takes its share of the cone work.
 template <int COV, bool F16, bool FP64SHELL, bool COUNT, int G>
 __global__ __launch_bounds__(256) void march_kernel(FrameConsts fc, FastViews fv, OutputViews ov, int full) {
   __shared__ float sjit[48];
+  __shared__ float skern[18];
+  __shared__ ConeLds cone_lds[4];
   if (threadIdx.x < 48) sjit[threadIdx.x] = (&fc.step_jitter[0][0])[threadIdx.x];
+  if (threadIdx.x < 18) skern[threadIdx.x] = (&fc.kern[0][0])[threadIdx.x];
   __syncthreads();
 
   constexpr int RPB = 256 / G;  // rays per workgroup
   constexpr int TX = 8, TY = RPB / 8;
   const int lane = threadIdx.x & 63;
-  const int slot = threadIdx.x / G;   // ray slot in the workgroup
-  const int j = threadIdx.x % G;      // step offset within a chunk
-  const int gbase = lane - j;         // first lane of this ray's group (within the wave)
+  const int wave = threadIdx.x >> 6;
+  const int slot = threadIdx.x / G;  // ray slot in the workgroup
+  const int j = threadIdx.x % G;     // step offset within a chunk
+  const int gbase = lane - j;        // first lane of this ray's group
   const int rx = blockIdx.x * TX + (slot % TX);
   const int ry = blockIdx.y * TY + (slot / TX);
   const int W = fc.width, H = fc.height;
@@ -318,26 +323,36 @@ __global__ __launch_bounds__(256) void march_kernel(FrameConsts fc, FastViews fv
   const int px = full ? rx : rx * 4 + pixelID / 4;
   const int py = full ? ry : ry * 4 + pixelID % 4;
   const bool active = px < W && py < H;
+  const int nls = fc.light_samples;
+  ConeLds& L = cone_lds[wave];
   unsigned long long n_steps = 0, n_cloud = 0, n_cone = 0, n_cone_er = 0, n_early = 0;
   unsigned long long n_ocean = 0, n_sky = 0, n_marched = 0;
+
+  // --- per-ray set-up (main, :690-757) --------------------------------------
+  RayMarch rm;
+  bool gdone = true;
+  f3 bg = mk(0.0f, 0.0f, 0.0f);
+  float4 color = make_float4(0.0f, 0.0f, 0.0f, 1.0f);
+  Ray ray;
+  ray.o = ld3(fc.eye);
+  ray.d = mk(0.0f, 1.0f, 0.0f);
   if (active) {
     float ux = (float)px / (float)W, uy = 1.0f - (float)py / (float)H;
-    Ray ray = cast_ray(fc, fc.view, ux, uy, fc.jit_march[0], fc.jit_march[1]);
+    ray = cast_ray(fc, fc.view, ux, uy, fc.jit_march[0], fc.jit_march[1]);
     const float d = ray.d.y;
-    float4 color, mask = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
     if (d < 0.0f) {
       float a = -ray.d.y * 5.5f;
       color = make_float4(mixf(0.0f * 0.4f, 0.0f * 0.5f, a), mixf(0.16f * 0.4f, 0.73f * 0.5f, a),
                           mixf(0.51f * 0.4f, 0.95f * 0.5f, a), 1.0f);
       n_ocean = 1;
     } else {
-      f3 bg = atmosphere(fc, ray.d) * fmaxf(fc.bg_min, d);
+      bg = atmosphere(fc, ray.d) * fmaxf(fc.bg_min, d);
       if (d < fc.fade_start) {
         color = make_float4(bg.x, bg.y, bg.z, 1.0f);
         n_sky = 1;
       } else {
         n_marched = 1;
-        RayMarch rm;
+        gdone = false;
         rm.ray = ray;
         rm.c = mk(ray.o.x, -fc.earth_r, ray.o.z);
         f3 startPos;
@@ -359,47 +374,113 @@ __global__ __launch_bounds__(256) void march_kernel(FrameConsts fc, FastViews fv
         rm.cone_scale = rm.stepSize * rm.inv_thick;
         rm.pixelID = pixelID;
         rm.cap = 4 * max(fc.steps_min, fc.steps_max) + 64;
-
-        float accum = 0.0f, transmittance = 1.0f, col = 0.0f;
-        bool exited = false;
-        for (int k0 = 0;; k0 += G) {
-          StepOut so = eval_step<COV, F16>(fc, fv, sjit, rm, k0 + j);
-          bool last_valid = false;
-#pragma unroll
-          for (int i = 0; i < G; ++i) {
-            const int src = gbase + i;
-            const bool v = G == 1 ? so.valid : (bool)__shfl((int)so.valid, src, 64);
-            const bool cl = G == 1 ? so.cloud : (bool)__shfl((int)so.cloud, src, 64);
-            const float da = G == 1 ? so.da : __shfl(so.da, src, 64);
-            const float E = G == 1 ? so.E : __shfl(so.E, src, 64);
-            const int ce = COUNT ? (G == 1 ? so.cone_er : __shfl(so.cone_er, src, 64)) : 0;
-            last_valid = v;
-            if (v && !exited) {
-              if (COUNT && j == 0) n_steps++;
-              if (cl) {
-                if (COUNT && j == 0) {
-                  n_cloud++;
-                  n_cone += (unsigned)fc.light_samples;
-                  n_cone_er += (unsigned)ce;
-                }
-                accum += da;
-                transmittance = mixf(transmittance, E, 1.0f - accum);
-                col += transmittance;
-              }
-              if (accum >= 1.0f) {
-                accum = 1.0f;
-                exited = true;
-                if (COUNT && j == 0) n_early++;
-              }
-            }
-          }
-          if (exited || !last_valid) break;
-        }
-        float god = accum;
-        accum = accum * smoothstepf(0.0f, 1.0f, fminf(1.0f, remap(ray.d.y, fc.fade_start, fc.fade_end, 0.0f, 1.0f)));
-        color = make_float4(mixf(bg.x, col, accum), mixf(bg.y, col, accum), mixf(bg.z, col, accum), 1.0f);
-        mask = encode_float_rgba(25.0f * fminf(0.05f, 1.0f - god));
       }
+    }
+  }
+
+  // --- rayMarch (:565-688), wave-uniform chunk loop ----------------------------
+  float accum = 0.0f, transmittance = 1.0f, col = 0.0f;
+  for (int k0 = 0; __any(!gdone); k0 += G) {
+    Prim pr;
+    pr.valid = false;
+    pr.cloud = false;
+    pr.da = 0.0f;
+    if (!gdone) pr = eval_primary<COV, F16>(fc, fv, sjit, rm, k0 + j);
+
+    // compacted cone march of this chunk's in-cloud steps
+    float E = 0.0f;
+    int cone_hits = 0;
+    const unsigned long long bal = __ballot(pr.cloud);
+    if (bal) {
+      const int ncl = __popcll(bal);
+      const int rank = __popcll(bal & ((1ull << lane) - 1ull));
+      if (pr.cloud) {
+        L.own[rank][0] = make_float4(pr.rel.x, pr.rel.y, pr.rel.z, rm.cone_scale);
+        L.own[rank][1] = make_float4(pr.sc, pr.st, pr.m, pr.inv1m);
+      }
+      __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+      __builtin_amdgcn_wave_barrier();
+      __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+      const int total = ncl * nls;
+      for (int w = lane; w < total; w += 64) {
+        const int r = w / nls, i = w - r * nls;
+        const float4 a = L.own[r][0], b = L.own[r][1];
+        const float s = (float)i * a.w;
+        const f3 slp = mk(a.x + skern[3 * i] * s, a.y + skern[3 * i + 1] * s, a.z + skern[3 * i + 2] * s);
+        float2 clf = sample_lf<F16>(fv.lf, fv.lf_sh, slp);
+        float2 cw = make_float2(0.0f, 0.0f);
+        if (COV == MTC_COVERAGE_WEATHER) cw = sample_map<F16>(fv.weather, fv.weather_sh, slp.x * fc.weather_scale,
+                                                               slp.z * fc.weather_scale);
+        const float cb = density<COV>(clf, cw, fc.coverage_const, b.x, b.y);
+        L.contrib[w] = cb > 0.0f ? (fc.cone_erosion_scale * cb - b.z) * b.w : 0.0f;
+        if (COUNT) L.hit[w] = cb > 0.0f ? 1 : 0;
+      }
+      __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+      __builtin_amdgcn_wave_barrier();
+      __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+      if (pr.cloud) {
+        float dal = 0.0f;
+        for (int i = 0; i < nls; ++i) {
+          dal += L.contrib[rank * nls + i];
+          if (COUNT) cone_hits += L.hit[rank * nls + i];
+        }
+        // GetLightEnergy (:331-388): exp(-dl) once for both identical terms
+        const float h = pr.h;
+        float prim = fast_exp(-dal);
+        float att = fmaxf(remap(rm.cos_angle, 0.7f, 1.0f, prim, prim * 0.25f), prim);
+        float depth = 0.05f + fast_pow(pr.base, clampf(remap(h * 0.125f, 0.3f, 0.85f, 0.5f, 2.0f), 0.5f, 2.0f));
+        float vert = fast_pow(clampf(remap(h * 1.5f, 0.07f, 0.34f, 0.1f, 1.0f), 0.1f, 1.0f), 0.8f);
+        E = att * prim * (depth * vert) * rm.HG_light * fc.brightness;
+      }
+      // the next chunk rewrites the work area
+      __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+      __builtin_amdgcn_wave_barrier();
+      __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+    }
+
+    // accumulation / early exit in step order (:674-684)
+    if (!gdone) {
+      bool last_valid = false, exited = false;
+#pragma unroll
+      for (int i = 0; i < G; ++i) {
+        const int src = gbase + i;
+        const bool v = G == 1 ? pr.valid : (bool)__shfl((int)pr.valid, src, 64);
+        const bool cl = G == 1 ? pr.cloud : (bool)__shfl((int)pr.cloud, src, 64);
+        const float da = G == 1 ? pr.da : __shfl(pr.da, src, 64);
+        const float e = G == 1 ? E : __shfl(E, src, 64);
+        const int ch = COUNT ? (G == 1 ? cone_hits : __shfl(cone_hits, src, 64)) : 0;
+        last_valid = v;
+        if (v && !exited) {
+          if (COUNT && j == 0) n_steps++;
+          if (cl) {
+            if (COUNT && j == 0) {
+              n_cloud++;
+              n_cone += (unsigned)nls;
+              n_cone_er += (unsigned)ch;
+            }
+            accum += da;
+            transmittance = mixf(transmittance, e, 1.0f - accum);
+            col += transmittance;
+          }
+          if (accum >= 1.0f) {
+            accum = 1.0f;
+            exited = true;
+            if (COUNT && j == 0) n_early++;
+          }
+        }
+      }
+      gdone = exited || !last_valid;
+    }
+  }
+
+  // --- composite and god-ray mask (:759-825) ---------------------------------
+  if (active) {
+    float4 mask = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+    if (n_marched) {
+      float god = accum;
+      float a = accum * smoothstepf(0.0f, 1.0f, fminf(1.0f, remap(ray.d.y, fc.fade_start, fc.fade_end, 0.0f, 1.0f)));
+      color = make_float4(mixf(bg.x, col, a), mixf(bg.y, col, a), mixf(bg.z, col, a), 1.0f);
+      mask = encode_float_rgba(25.0f * fminf(0.05f, 1.0f - god));
     }
     if (j == 0) {
       const size_t o = (size_t)py * (size_t)W + (size_t)px;

@@ -109,13 +109,103 @@ __global__ __launch_bounds__(256) void reproject_kernel(FrameConsts fc, OutputVi
   }
 }
 
+// Tiled variant for the optimised frame path.  The taps of a pixel lie on
+// the segment from old_uv (tap 0) towards uv (tap N-1), and their rounded,
+// clamped indices are monotone along it, so the taps of a 64x4 tile fall in
+// the bounding box of the tiles' first and last taps.  That box is staged
+// in LDS with coalesced float4 loads and the taps gather from LDS; a box
+// larger than the LDS tile falls back to global gathers.  Tap indices and
+// summation order are those of reproject_kernel, so the output is
+// bit-identical.  Pixels the march of this frame overwrites are skipped.
+constexpr int kTileCap = 1536;  // float4 texels staged per workgroup (24 KiB)
+
+template <bool SKIP_SELECTED>
+__global__ __launch_bounds__(256) void reproject_tiled_kernel(FrameConsts fc, OutputViews ov) {
+  __shared__ float4 tile[kTileCap];
+  __shared__ int box[4];
+  const int x = blockIdx.x * 64 + (threadIdx.x & 63);
+  const int y = blockIdx.y * 4 + (threadIdx.x >> 6);
+  const int W = fc.width, H = fc.height;
+  const int N = fc.blur_samples;
+  bool live = x < W && y < H;
+  if (SKIP_SELECTED && live && (x & 3) == fc.frame / 4 && (y & 3) == fc.frame % 4) live = false;
+  if (threadIdx.x == 0) {
+    box[0] = 0x7fffffff;
+    box[1] = 0x7fffffff;
+    box[2] = -1;
+    box[3] = -1;
+  }
+  __syncthreads();
+  float ou = 0.0f, ovv = 0.0f, mx = 0.0f, my = 0.0f;
+  const float fW = (float)W, fH = (float)H, maxx = (float)(W - 1), maxy = (float)(H - 1);
+  if (live) {
+    float ux = (float)x / (float)W, uy = (float)y / (float)H;
+    Ray ray = cast_ray(fc, ux, uy);
+    f3 c = ray.o;
+    c.y = -fc.earth_r;
+    f3 P = inner_hit(ray.o, ray.d, c, fc.r_in);
+    const float* m = fc.old_view;
+    f3 q = mk(((m[0] * P.x + m[4] * P.y) + m[8] * P.z) + m[12] * 1.0f,
+              ((m[1] * P.x + m[5] * P.y) + m[9] * P.z) + m[13] * 1.0f,
+              ((m[2] * P.x + m[6] * P.y) + m[10] * P.z) + m[14] * 1.0f);
+    f3 od = normalize(q);
+    float nz = -od.z;
+    od = od / nz;
+    ou = (od.x / fc.tfx) * 0.5f + 0.5f;
+    ovv = (od.y / fc.tfy) * 0.5f + 0.5f;
+    mx = ou - ux;
+    my = ovv - uy;
+    // first and last taps bound all of them
+    int x0 = (int)fminf(fmaxf(rintf(ou * fW), 0.0f), maxx);
+    int y0 = (int)fminf(fmaxf(rintf(ovv * fH), 0.0f), maxy);
+    float fr = (float)(N - 1) / (float)N;
+    int x1 = (int)fminf(fmaxf(rintf((ou - mx * fr) * fW), 0.0f), maxx);
+    int y1 = (int)fminf(fmaxf(rintf((ovv - my * fr) * fH), 0.0f), maxy);
+    atomicMin(&box[0], min(x0, x1));
+    atomicMin(&box[1], min(y0, y1));
+    atomicMax(&box[2], max(x0, x1));
+    atomicMax(&box[3], max(y0, y1));
+  }
+  __syncthreads();
+  const int bx0 = box[0], by0 = box[1];
+  const int bw = box[2] - bx0 + 1, bh = box[3] - by0 + 1;
+  const bool staged = box[2] >= 0 && bw * bh <= kTileCap;
+  if (staged) {
+    for (int e = threadIdx.x; e < bw * bh; e += 256) {
+      int ry = e / bw;
+      tile[e] = ov.prev[(size_t)(by0 + ry) * W + (bx0 + (e - ry * bw))];
+    }
+  }
+  __syncthreads();
+  if (!live) return;
+  float4 acc = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+  for (int i = 0; i < N; ++i) {
+    float fr = (float)i / (float)N;
+    float bxo = mx * fr, byo = my * fr;
+    float iu = rintf((ou - bxo) * fW);
+    float iv = rintf((ovv - byo) * fH);
+    int sx = (int)fminf(fmaxf(iu, 0.0f), maxx);
+    int sy = (int)fminf(fmaxf(iv, 0.0f), maxy);
+    float4 t = staged ? tile[(sy - by0) * bw + (sx - bx0)] : ov.prev[sy * W + sx];
+    acc.x += t.x;
+    acc.y += t.y;
+    acc.z += t.z;
+    acc.w += t.w;
+  }
+  const float fN = (float)N;
+  ov.color[(size_t)y * (size_t)W + (size_t)x] = make_float4(acc.x / fN, acc.y / fN, acc.z / fN, acc.w / fN);
+}
+
 }  // namespace reproj
 }  // namespace mtc
 
-hipError_t launch_reproject(const FrameConsts& fc, const OutputViews& ov, bool indices, hipStream_t stream) {
+// mode 0: plain kernel; 1: tap-index map; 2: LDS-tiled kernel that skips
+// the pixels the march overwrites
+hipError_t launch_reproject(const FrameConsts& fc, const OutputViews& ov, int mode, hipStream_t stream) {
   dim3 grid((fc.width + 63) / 64, (fc.height + 3) / 4);
   dim3 block(256);
-  if (indices) hipLaunchKernelGGL((mtc::reproj::reproject_kernel<true>), grid, block, 0, stream, fc, ov);
+  if (mode == 1) hipLaunchKernelGGL((mtc::reproj::reproject_kernel<true>), grid, block, 0, stream, fc, ov);
+  else if (mode == 2) hipLaunchKernelGGL((mtc::reproj::reproject_tiled_kernel<true>), grid, block, 0, stream, fc, ov);
   else hipLaunchKernelGGL((mtc::reproj::reproject_kernel<false>), grid, block, 0, stream, fc, ov);
   return hipGetLastError();
 }

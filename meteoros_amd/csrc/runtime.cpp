@@ -31,7 +31,7 @@ hipError_t launch_march_fast(const FrameConsts& fc, const uint4* lf, const uint4
                              int lanes_per_ray, hipStream_t stream);
 hipError_t pack_fast_resources(const VolumeViews& vv, uint4* lf, uint4* hf, uint4* curl, uint4* weather,
                                hipStream_t s);
-hipError_t launch_reproject(const FrameConsts& fc, const OutputViews& ov, bool indices, hipStream_t stream);
+hipError_t launch_reproject(const FrameConsts& fc, const OutputViews& ov, int mode, hipStream_t stream);
 hipError_t launch_noise(int kind, void* out, int N, int fmt, uint32_t seed, hipStream_t stream);
 
 namespace {
@@ -352,7 +352,9 @@ int mtc_frame(mtc_ctx* c, const mtc_camera* cur, const mtc_camera* old, const mt
   }
   if (!full) {
     if (timing) HIP_TRY(hipEventRecord(ev[0], s), "mtc_frame: event");
-    HIP_TRY(launch_reproject(fc, ov, false, s), "reprojection kernel launch");
+    // the optimised kernels pair with the LDS-tiled reprojection; the plain
+    // kernel keeps the one-lane-per-pixel gather
+    HIP_TRY(launch_reproject(fc, ov, c->desc.kernel == MTC_KERNEL_PLAIN ? 0 : 2, s), "reprojection kernel launch");
   }
   if (timing) HIP_TRY(hipEventRecord(ev[1], s), "mtc_frame: event");
   rc = launch_march(c, fc, ov, full, s);
@@ -495,7 +497,7 @@ int mtc_reprojection_indices(mtc_ctx* c, const mtc_camera* cur, const mtc_camera
   OutputViews ov;
   std::memset(&ov, 0, sizeof(ov));
   ov.reproj_idx = d;
-  hipError_t e = launch_reproject(fc, ov, true, c->stream);
+  hipError_t e = launch_reproject(fc, ov, 1, c->stream);
   if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
   if (e == hipSuccess) e = hipMemcpy(idx, d, need * sizeof(int32_t), hipMemcpyDeviceToHost);
   (void)hipFree(d);
