@@ -318,6 +318,43 @@ int mtc_reprojection_indices(mtc_ctx* ctx, const mtc_camera* cur, const mtc_came
                              const mtc_time* time, const mtc_cloud_params* params,
                              int32_t* idx, size_t n_idx);
 
+/* ------------------------------------------------------------------ */
+/* batched views: many views of uneven size in one persistent launch  */
+/* (no reference counterpart; the reference renders one view).        */
+/* ------------------------------------------------------------------ */
+typedef struct mtc_batch mtc_batch;
+
+/* Launch shape of a batch; chosen by mtc_batch_autotune or by the caller. */
+typedef struct mtc_batch_tuning {
+  int32_t lanes_per_ray;   /* 4 or 8 lanes cooperate on one ray          */
+  int32_t tile_rays_x;     /* tile width in rays; height = 256 / lanes / width */
+  int32_t blocks_per_cu;   /* persistent workgroups per compute unit     */
+  int32_t order;           /* 0: work items by descending view cost (steps x
+                              rays), 1: in view order                    */
+} mtc_batch_tuning;
+
+/* A batch of n_views views (sizes fixed at creation) rendered with ctx's
+ * volumes; ctx must be a MTC_KERNEL_FAST or MTC_KERNEL_PRECISION context.
+ * Every pixel of every view is marched (no reprojection). */
+int mtc_batch_create(mtc_ctx* ctx, int32_t n_views, const int32_t* widths, const int32_t* heights,
+                     mtc_batch** out);
+int mtc_batch_destroy(mtc_batch* b);
+int mtc_batch_set_tuning(mtc_batch* b, const mtc_batch_tuning* t);
+int mtc_batch_get_tuning(mtc_batch* b, mtc_batch_tuning* t);
+/* Render all views: per-view uniform arrays of length n_views.  All views
+ * must share coverage_mode.  The launch sequence (queue reset + persistent
+ * kernel) is captured once per tuning into a hipGraph and replayed. */
+int mtc_batch_render(mtc_batch* b, const mtc_camera* cams, const mtc_time* times, const mtc_sun* suns,
+                     const mtc_cloud_params* params, void* stream);
+/* Time candidate tunings on these inputs and keep the fastest; *best_ms is
+ * its render time (median of the timed repeats). */
+int mtc_batch_autotune(mtc_batch* b, const mtc_camera* cams, const mtc_time* times, const mtc_sun* suns,
+                       const mtc_cloud_params* params, mtc_batch_tuning* best, double* best_ms);
+/* Wall time of the last render measured with hipEvents (waits for it). */
+int mtc_batch_last_ms(mtc_batch* b, double* ms);
+/* Copy one view's RGBA32F colour and mask (W*H*4 floats each; either may be NULL). */
+int mtc_batch_read(mtc_batch* b, int32_t view, float* rgba, float* mask, size_t n_floats);
+
 #ifdef __cplusplus
 }
 #endif

@@ -100,6 +100,11 @@ class MtcTimings(C.Structure):
     _fields_ = [("frames", C.c_uint64), ("reproject_ms", C.c_double), ("march_ms", C.c_double)]
 
 
+class MtcBatchTuning(C.Structure):
+    _fields_ = [("lanes_per_ray", C.c_int32), ("tile_rays_x", C.c_int32), ("blocks_per_cu", C.c_int32),
+                ("order", C.c_int32)]
+
+
 P = C.POINTER
 # (name, restype, argtypes) of every exported entry point of meteoros_cloud.h
 SIGNATURES = [
@@ -138,6 +143,15 @@ SIGNATURES = [
     ("mtc_reset_timings", C.c_int, [C.c_void_p]),
     ("mtc_reprojection_indices", C.c_int, [C.c_void_p, P(MtcCamera), P(MtcCamera), P(MtcTime), P(MtcCloudParams),
                                            C.c_void_p, C.c_size_t]),
+    ("mtc_batch_create", C.c_int, [C.c_void_p, C.c_int32, P(C.c_int32), P(C.c_int32), P(C.c_void_p)]),
+    ("mtc_batch_destroy", C.c_int, [C.c_void_p]),
+    ("mtc_batch_set_tuning", C.c_int, [C.c_void_p, P(MtcBatchTuning)]),
+    ("mtc_batch_get_tuning", C.c_int, [C.c_void_p, P(MtcBatchTuning)]),
+    ("mtc_batch_render", C.c_int, [C.c_void_p, P(MtcCamera), P(MtcTime), P(MtcSun), P(MtcCloudParams), C.c_void_p]),
+    ("mtc_batch_autotune", C.c_int, [C.c_void_p, P(MtcCamera), P(MtcTime), P(MtcSun), P(MtcCloudParams),
+                                     P(MtcBatchTuning), P(C.c_double)]),
+    ("mtc_batch_last_ms", C.c_int, [C.c_void_p, P(C.c_double)]),
+    ("mtc_batch_read", C.c_int, [C.c_void_p, C.c_int32, C.c_void_p, C.c_void_p, C.c_size_t]),
 ]
 
 _lib = None

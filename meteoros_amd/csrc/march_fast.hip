@@ -301,23 +301,17 @@ struct ConeLds {
 // so the result equals the sequential march.  The loop is wave-uniform: a
 // lane whose ray has finished still takes its share of the cone work.
 template <int COV, bool F16, bool FP64SHELL, bool COUNT, int G>
-__global__ __launch_bounds__(256) void march_kernel(FrameConsts fc, FastViews fv, OutputViews ov, int full) {
-  __shared__ float sjit[48];
-  __shared__ float skern[18];
-  __shared__ ConeLds cone_lds[4];
-  if (threadIdx.x < 48) sjit[threadIdx.x] = (&fc.step_jitter[0][0])[threadIdx.x];
-  if (threadIdx.x < 18) skern[threadIdx.x] = (&fc.kern[0][0])[threadIdx.x];
-  __syncthreads();
-
-  constexpr int RPB = 256 / G;  // rays per workgroup
-  constexpr int TX = 8, TY = RPB / 8;
+__device__ __forceinline__ void march_tile(const FrameConsts& fc, const FastViews& fv, float4* __restrict__ out_color,
+                                           float4* __restrict__ out_mask, unsigned long long* counters, int full,
+                                           int rx0, int ry0, int tile_x, const float* sjit, const float* skern,
+                                           ConeLds* cone_lds) {
   const int lane = threadIdx.x & 63;
   const int wave = threadIdx.x >> 6;
-  const int slot = threadIdx.x / G;  // ray slot in the workgroup
+  const int slot = threadIdx.x / G;  // ray slot in the tile
   const int j = threadIdx.x % G;     // step offset within a chunk
   const int gbase = lane - j;        // first lane of this ray's group
-  const int rx = blockIdx.x * TX + (slot % TX);
-  const int ry = blockIdx.y * TY + (slot / TX);
+  const int rx = rx0 + (slot % tile_x);
+  const int ry = ry0 + (slot / tile_x);
   const int W = fc.width, H = fc.height;
   const int pixelID = fc.frame;
   const int px = full ? rx : rx * 4 + pixelID / 4;
@@ -484,22 +478,70 @@ __global__ __launch_bounds__(256) void march_kernel(FrameConsts fc, FastViews fv
     }
     if (j == 0) {
       const size_t o = (size_t)py * (size_t)W + (size_t)px;
-      ov.color[o] = color;
-      ov.mask[o] = mask;
+      out_color[o] = color;
+      out_mask[o] = mask;
     }
   }
   if (COUNT) {
     if (j != 0) n_ocean = n_sky = n_marched = 0;
-    wave_count(ov.counters + 0, n_marched);
-    wave_count(ov.counters + 1, n_ocean);
-    wave_count(ov.counters + 2, n_sky);
-    wave_count(ov.counters + 3, n_steps);
-    wave_count(ov.counters + 4, n_cloud);
-    wave_count(ov.counters + 5, n_cone);
-    wave_count(ov.counters + 6, n_cone_er);
-    wave_count(ov.counters + 7, n_early);
+    wave_count(counters + 0, n_marched);
+    wave_count(counters + 1, n_ocean);
+    wave_count(counters + 2, n_sky);
+    wave_count(counters + 3, n_steps);
+    wave_count(counters + 4, n_cloud);
+    wave_count(counters + 5, n_cone);
+    wave_count(counters + 6, n_cone_er);
+    wave_count(counters + 7, n_early);
   }
 }
+
+// One frame: a 256-lane workgroup marches an 8 x (32 / G) tile of rays.
+template <int COV, bool F16, bool FP64SHELL, bool COUNT, int G>
+__global__ __launch_bounds__(256) void march_kernel(FrameConsts fc, FastViews fv, OutputViews ov, int full) {
+  __shared__ float sjit[48];
+  __shared__ float skern[18];
+  __shared__ ConeLds cone_lds[4];
+  if (threadIdx.x < 48) sjit[threadIdx.x] = (&fc.step_jitter[0][0])[threadIdx.x];
+  if (threadIdx.x < 18) skern[threadIdx.x] = (&fc.kern[0][0])[threadIdx.x];
+  __syncthreads();
+  constexpr int TX = 8, TY = 256 / G / 8;
+  march_tile<COV, F16, FP64SHELL, COUNT, G>(fc, fv, ov.color, ov.mask, ov.counters, full, blockIdx.x * TX,
+                                            blockIdx.y * TY, TX, sjit, skern, cone_lds);
+}
+
+// Many views in one launch: a persistent grid pulls (view, tile) work items
+// from a queue (items sorted by estimated cost when the batch was built).
+// Each view carries its own FrameConsts, size, output offset and steps.
+template <int COV, bool F16, int G>
+__global__ __launch_bounds__(256) void march_batch_kernel(const BatchView* __restrict__ views,
+                                                          const BatchItem* __restrict__ items, int n_items,
+                                                          unsigned int* __restrict__ queue, FastViews fv,
+                                                          float4* __restrict__ color, float4* __restrict__ mask,
+                                                          int tile_x) {
+  __shared__ float sjit[48];
+  __shared__ float skern[18];
+  __shared__ ConeLds cone_lds[4];
+  __shared__ int s_item;
+  const int tile_y = 256 / G / tile_x;
+  for (;;) {
+    if (threadIdx.x == 0) s_item = (int)atomicAdd(queue, 1u);
+    __syncthreads();
+    const int it = s_item;
+    if (it >= n_items) break;
+    const BatchItem wi = items[it];
+    const BatchView& bv = views[__builtin_amdgcn_readfirstlane(wi.view)];
+    if (threadIdx.x < 48) sjit[threadIdx.x] = (&bv.fc.step_jitter[0][0])[threadIdx.x];
+    if (threadIdx.x < 18) skern[threadIdx.x] = (&bv.fc.kern[0][0])[threadIdx.x];
+    __syncthreads();
+    const int tx = wi.tile % bv.tiles_x, ty = wi.tile / bv.tiles_x;
+    march_tile<COV, F16, true, false, G>(bv.fc, fv, color + bv.offset, mask + bv.offset, nullptr, 1, tx * tile_x,
+                                         ty * tile_y, tile_x, sjit, skern, cone_lds);
+    // the next item rewrites the tables and the queue slot
+    __syncthreads();
+  }
+}
+
+__global__ void reset_queue_kernel(unsigned int* queue) { *queue = 0u; }
 
 // ---------------------------------------------------------------------------
 // footprint packing (run once per volume upload)
@@ -632,5 +674,36 @@ hipError_t launch_march_fast(const FrameConsts& fc, const uint4* lf, const uint4
   bool ok = variant == 0 ? launch_g<false, true>(lanes_per_ray, stream, fc, fv, ov, full)
                          : launch_g<true, true>(lanes_per_ray, stream, fc, fv, ov, full);
   if (!ok) return hipErrorInvalidValue;
+  return hipGetLastError();
+}
+
+template <int COV, bool F16, int G>
+static void launch_batch_g(const BatchView* views, const BatchItem* items, int n_items, unsigned int* queue,
+                           const mtc::fast::FastViews& fv, float4* color, float4* mask, int tile_x, int blocks,
+                           hipStream_t s) {
+  hipLaunchKernelGGL(mtc::fast::reset_queue_kernel, dim3(1), dim3(1), 0, s, queue);
+  hipLaunchKernelGGL((mtc::fast::march_batch_kernel<COV, F16, G>), dim3(blocks), dim3(256), 0, s, views, items,
+                     n_items, queue, fv, color, mask, tile_x);
+}
+
+// Enqueue one batched render: queue reset + persistent march over the items.
+// variant 0 = fast, 1 = precision; lanes_per_ray 4 or 8.
+hipError_t launch_march_batch(const BatchView* views, const BatchItem* items, int n_items, unsigned int* queue,
+                              const uint4* lf, const uint4* hf, const uint4* curl, const uint4* weather,
+                              const VolumeViews& vv, float4* color, float4* mask, int coverage_mode, int variant,
+                              int lanes_per_ray, int tile_x, int blocks, hipStream_t s) {
+  mtc::fast::FastViews fv{lf, hf, curl, weather, vv.lf_shift, vv.hf_shift, vv.curl_shift, vv.weather_shift};
+  const bool w = coverage_mode == MTC_COVERAGE_WEATHER;
+#define MTC_BATCH(COVV, F16V, GV) launch_batch_g<COVV, F16V, GV>(views, items, n_items, queue, fv, color, mask, tile_x, blocks, s)
+  if (lanes_per_ray == 8) {
+    if (variant == 0) { if (w) MTC_BATCH(MTC_COVERAGE_WEATHER, false, 8); else MTC_BATCH(MTC_COVERAGE_CONSTANT, false, 8); }
+    else { if (w) MTC_BATCH(MTC_COVERAGE_WEATHER, true, 8); else MTC_BATCH(MTC_COVERAGE_CONSTANT, true, 8); }
+  } else if (lanes_per_ray == 4) {
+    if (variant == 0) { if (w) MTC_BATCH(MTC_COVERAGE_WEATHER, false, 4); else MTC_BATCH(MTC_COVERAGE_CONSTANT, false, 4); }
+    else { if (w) MTC_BATCH(MTC_COVERAGE_WEATHER, true, 4); else MTC_BATCH(MTC_COVERAGE_CONSTANT, true, 4); }
+  } else {
+    return hipErrorInvalidValue;
+  }
+#undef MTC_BATCH
   return hipGetLastError();
 }

@@ -59,32 +59,6 @@ int mtc_fail(int code, const std::string& msg) {
     }                                                                                                  \
   } while (0)
 
-struct mtc_ctx {
-  mtc_desc desc;
-  hipStream_t stream = nullptr;
-  void* lf = nullptr;
-  void* hf = nullptr;
-  uint32_t* curl = nullptr;
-  uint32_t* weather = nullptr;
-  float4* img[2] = {nullptr, nullptr};
-  float4* mask = nullptr;
-  unsigned long long* counters = nullptr;
-  // footprint-packed copies of the resources read by the fast kernels
-  uint4* q_lf = nullptr;
-  uint4* q_hf = nullptr;
-  uint4* q_curl = nullptr;
-  uint4* q_weather = nullptr;
-  int latest = 0;
-  int lanes_per_ray = 8;  // fast kernels: lanes cooperating on one ray
-  bool volumes_ready = false;
-  // MTC_FLAG_TIMING: event triples (before reprojection, between, after
-  // march) of frames not yet folded into the totals
-  std::vector<std::array<hipEvent_t, 3>> pending;
-  std::vector<hipEvent_t> event_pool;
-  mtc_timings totals = {0, 0.0, 0.0};
-  size_t lf_bytes = 0, hf_bytes = 0, curl_bytes = 0, weather_bytes = 0;
-};
-
 namespace {
 
 size_t texel_bytes(int fmt) { return fmt == MTC_FMT_FP16 ? 8 : 4; }
@@ -144,23 +118,6 @@ void release(mtc_ctx* c) {
   delete c;
 }
 
-VolumeViews volume_views(const mtc_ctx* c) {
-  VolumeViews v;
-  v.lf = c->lf;
-  v.hf = c->hf;
-  v.curl = c->curl;
-  v.weather = c->weather;
-  v.lf_dim = c->desc.volumes.lf_dim;
-  v.hf_dim = c->desc.volumes.hf_dim;
-  v.curl_dim = c->desc.volumes.curl_dim;
-  v.weather_dim = c->desc.volumes.weather_dim;
-  v.lf_shift = log2i(v.lf_dim);
-  v.hf_shift = log2i(v.hf_dim);
-  v.curl_shift = log2i(v.curl_dim);
-  v.weather_shift = log2i(v.weather_dim);
-  v.format = c->desc.volumes.format;
-  return v;
-}
 
 hipStream_t pick_stream(mtc_ctx* c, void* s) { return s ? (hipStream_t)s : c->stream; }
 
@@ -189,6 +146,24 @@ int launch_march(mtc_ctx* c, const FrameConsts& fc, const OutputViews& ov, int f
 }
 
 }  // namespace
+
+VolumeViews volume_views(const mtc_ctx* c) {
+  VolumeViews v;
+  v.lf = c->lf;
+  v.hf = c->hf;
+  v.curl = c->curl;
+  v.weather = c->weather;
+  v.lf_dim = c->desc.volumes.lf_dim;
+  v.hf_dim = c->desc.volumes.hf_dim;
+  v.curl_dim = c->desc.volumes.curl_dim;
+  v.weather_dim = c->desc.volumes.weather_dim;
+  v.lf_shift = log2i(v.lf_dim);
+  v.hf_shift = log2i(v.hf_dim);
+  v.curl_shift = log2i(v.curl_dim);
+  v.weather_shift = log2i(v.weather_dim);
+  v.format = c->desc.volumes.format;
+  return v;
+}
 
 extern "C" {
 

@@ -2,8 +2,10 @@
 #pragma once
 #include <hip/hip_runtime.h>
 
+#include <array>
 #include <cstdint>
 #include <string>
+#include <vector>
 
 #include "meteoros_cloud.h"
 
@@ -64,6 +66,18 @@ struct VolumeViews {
   int32_t format;
 };
 
+// Batched rendering: one work item = one tile of one view.
+struct BatchItem {
+  int view;
+  int tile;
+};
+struct BatchView {
+  FrameConsts fc;
+  unsigned long long offset;  // first texel of the view in the packed outputs
+  int tiles_x;                // tiles per row of marched rays
+  int pad;
+};
+
 struct OutputViews {
   float4* color;      // current result image, W*H RGBA32F
   const float4* prev; // previous result image
@@ -71,3 +85,33 @@ struct OutputViews {
   unsigned long long* counters;  // 10 x u64 or nullptr
   int32_t* reproj_idx;           // optional tap-index map
 };
+
+// The context behind the opaque mtc_ctx handle.
+struct mtc_ctx {
+  mtc_desc desc;
+  hipStream_t stream = nullptr;
+  void* lf = nullptr;
+  void* hf = nullptr;
+  uint32_t* curl = nullptr;
+  uint32_t* weather = nullptr;
+  float4* img[2] = {nullptr, nullptr};
+  float4* mask = nullptr;
+  unsigned long long* counters = nullptr;
+  // footprint-packed copies of the resources read by the fast kernels
+  uint4* q_lf = nullptr;
+  uint4* q_hf = nullptr;
+  uint4* q_curl = nullptr;
+  uint4* q_weather = nullptr;
+  int latest = 0;
+  int lanes_per_ray = 8;  // fast kernels: lanes cooperating on one ray
+  bool volumes_ready = false;
+  // MTC_FLAG_TIMING: event triples (before reprojection, between, after
+  // march) of frames not yet folded into the totals
+  std::vector<std::array<hipEvent_t, 3>> pending;
+  std::vector<hipEvent_t> event_pool;
+  mtc_timings totals = {0, 0.0, 0.0};
+  size_t lf_bytes = 0, hf_bytes = 0, curl_bytes = 0, weather_bytes = 0;
+};
+
+// Device views of a context's base resources.
+VolumeViews volume_views(const mtc_ctx* c);

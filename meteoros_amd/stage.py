@@ -84,6 +84,14 @@ def preset_params(preset: int):
     return p, s
 
 
+def sun_from_angles(elevation_deg: float, azimuth_deg: float, sun: A.MtcSun, params: A.MtcCloudParams | None = None,
+                    distance_m: float = 1.0e8) -> None:
+    """Place the lighting sun (and the background sun of `params`) at an
+    elevation / azimuth (0 = -z) and distance."""
+    check(lib().mtc_sun_from_angles(float(elevation_deg), float(azimuth_deg), float(distance_m), C.byref(sun),
+                                    C.byref(params) if params is not None else None), "mtc_sun_from_angles")
+
+
 @dataclass
 class VolumeSpec:
     lf_dim: int = 128
@@ -209,3 +217,72 @@ class CloudStage:
         check(lib().mtc_reprojection_indices(self._ctx, C.byref(cam), C.byref(old), C.byref(time), C.byref(params),
                                              idx.ctypes.data, n), "mtc_reprojection_indices")
         return idx.reshape(self.height, self.width, int(params.motion_blur_samples))
+
+
+class Batch:
+    """Many views of uneven size rendered by one persistent launch (every
+    pixel marched); the launch is a hipGraph replayed per render."""
+
+    def __init__(self, stage: CloudStage, sizes):
+        self.stage = stage
+        self.sizes = [(int(w), int(h)) for w, h in sizes]
+        n = len(self.sizes)
+        ws = (C.c_int32 * n)(*[w for w, _ in self.sizes])
+        hs = (C.c_int32 * n)(*[h for _, h in self.sizes])
+        self._b = C.c_void_p()
+        check(lib().mtc_batch_create(stage.handle, n, ws, hs, C.byref(self._b)), "mtc_batch_create")
+
+    def close(self) -> None:
+        if self._b:
+            lib().mtc_batch_destroy(self._b)
+            self._b = C.c_void_p()
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    @staticmethod
+    def _arrays(cams, times, suns, params):
+        n = len(cams)
+        return ((A.MtcCamera * n)(*cams), (A.MtcTime * n)(*times), (A.MtcSun * n)(*suns),
+                (A.MtcCloudParams * n)(*params))
+
+    def render(self, cams, times, suns, params, stream=None) -> None:
+        check(lib().mtc_batch_render(self._b, *self._arrays(cams, times, suns, params), stream), "mtc_batch_render")
+
+    def autotune(self, cams, times, suns, params):
+        best = A.MtcBatchTuning()
+        ms = C.c_double()
+        check(lib().mtc_batch_autotune(self._b, *self._arrays(cams, times, suns, params), C.byref(best), C.byref(ms)),
+              "mtc_batch_autotune")
+        return {n: int(getattr(best, n)) for n, _ in A.MtcBatchTuning._fields_}, float(ms.value)
+
+    def set_tuning(self, lanes_per_ray=8, tile_rays_x=8, blocks_per_cu=4, order=0) -> None:
+        t = A.MtcBatchTuning(lanes_per_ray, tile_rays_x, blocks_per_cu, order)
+        check(lib().mtc_batch_set_tuning(self._b, C.byref(t)), "mtc_batch_set_tuning")
+
+    def tuning(self) -> dict:
+        t = A.MtcBatchTuning()
+        check(lib().mtc_batch_get_tuning(self._b, C.byref(t)), "mtc_batch_get_tuning")
+        return {n: int(getattr(t, n)) for n, _ in A.MtcBatchTuning._fields_}
+
+    def last_ms(self) -> float:
+        ms = C.c_double()
+        check(lib().mtc_batch_last_ms(self._b, C.byref(ms)), "mtc_batch_last_ms")
+        return float(ms.value)
+
+    def read(self, view: int):
+        w, h = self.sizes[view]
+        rgba = np.empty(w * h * 4, dtype=np.float32)
+        mask = np.empty(w * h * 4, dtype=np.float32)
+        check(lib().mtc_batch_read(self._b, int(view), rgba.ctypes.data, mask.ctypes.data, rgba.size),
+              "mtc_batch_read")
+        return rgba.reshape(h, w, 4), mask.reshape(h, w, 4)
