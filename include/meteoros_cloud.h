@@ -212,8 +212,8 @@ typedef struct mtc_noise_desc {
 enum {
   MTC_KERNEL_PLAIN = 0,     /* one lane per ray, FP32, GLSL op order          */
   MTC_KERNEL_FAST = 1,      /* FP16 texel cache + hoisted cone, FP32 accum    */
-  MTC_KERNEL_PRECISION = 2, /* FP16 volumes, BF16 density math, FP64 shell     */
-  MTC_KERNEL_LEARNED = 3    /* FAST with the cone march replaced by the MLP    */
+  MTC_KERNEL_PRECISION = 2, /* FP16 volumes, packed-FP16 filtering, FP64 shell */
+  MTC_KERNEL_LEARNED = 3    /* FAST with the cone march answered by the MLP   */
 };
 
 enum {
@@ -245,6 +245,9 @@ typedef struct mtc_counters {
   uint64_t early_exits;      /* accumDensity >= 1 breaks              */
   uint64_t mlp_queries;      /* learned-light queries                 */
   uint64_t mlp_fallbacks;    /* learned-light queries sent to the cone */
+  uint64_t mlp_audits;       /* network answers also cone-marched     */
+  uint64_t mlp_audit_sq_err; /* sum over audits of (network - cone)^2
+                                optical depth, in units of 1e-6       */
 } mtc_counters;
 
 /* Kernel time accumulated with hipEvents on the launch stream
@@ -319,6 +322,26 @@ int mtc_reprojection_indices(mtc_ctx* ctx, const mtc_camera* cur, const mtc_came
                              int32_t* idx, size_t n_idx);
 
 /* ------------------------------------------------------------------ */
+/* post-process passes that consume the stage's output                 */
+/* ------------------------------------------------------------------ */
+enum {
+  MTC_DISPLAY_RGBA8 = 0,   /* RGBA8 UNORM, the tone-map shader's rgba8 store */
+  MTC_DISPLAY_RGBA32F = 1  /* the same values before 8-bit quantisation       */
+};
+
+/* God-ray radial blur (postProcess_GodRays.frag:66-150; the reference
+ * records it but leaves it commented out, Renderer.cpp:826-832): adds
+ * 100 samples of the decoded god-ray mask along the pixel-to-sun line to
+ * the latest result image, in place.  Uses camera view/proj/eye and
+ * sun->light_color.  A sun behind the camera leaves the image unchanged. */
+int mtc_god_rays(mtc_ctx* ctx, const mtc_camera* cam, const mtc_sun* sun, void* stream);
+/* Uncharted-2 tone map + Wang-hash dither (postProcess_ToneMap.frag:68-84)
+ * of the latest result image into the context's display image. */
+int mtc_tone_map(mtc_ctx* ctx, const mtc_time* time, int32_t format, void* stream);
+/* Copy the display image (W*H*4 bytes for RGBA8, W*H*16 for RGBA32F). */
+int mtc_read_display(mtc_ctx* ctx, void* out, size_t n_bytes);
+
+/* ------------------------------------------------------------------ */
 /* batched views: many views of uneven size in one persistent launch  */
 /* (no reference counterpart; the reference renders one view).        */
 /* ------------------------------------------------------------------ */
@@ -354,6 +377,91 @@ int mtc_batch_autotune(mtc_batch* b, const mtc_camera* cams, const mtc_time* tim
 int mtc_batch_last_ms(mtc_batch* b, double* ms);
 /* Copy one view's RGBA32F colour and mask (W*H*4 floats each; either may be NULL). */
 int mtc_batch_read(mtc_batch* b, int32_t view, float* rgba, float* mask, size_t n_floats);
+
+/* ------------------------------------------------------------------ */
+/* learned light: an MLP for the cone light march (no reference        */
+/* counterpart; it stands in for GetLightEnergy's optical depth,       */
+/* cloudRayMarch.comp:338-373)                                         */
+/* ------------------------------------------------------------------ */
+/* Network 9 -> 64 -> 64 -> 64 -> 1 with SiLU.  Parameters are one FP32
+ * vector: W1[9][64], b1[64], W2[64][64], b2, W3[64][64], b3, w4[64], b4
+ * (W[k][n]: input k, output n).  Inputs of an in-cloud step: position in
+ * the shell (x, z modulo the weather tile, y modulo the LF tile), height
+ * fraction, local density, sun direction (3), cos of the view ray's
+ * zenith angle.  Output: log transmittance of the light march, -sum of
+ * the cone's density terms. */
+enum { MTC_MLP_INPUTS = 9, MTC_MLP_PARAMS = 9025 };
+
+typedef struct mtc_mlp mtc_mlp;
+
+typedef struct mtc_mlp_train_desc {
+  int32_t steps;         /* Adam steps                                   */
+  int32_t batch;         /* minibatch size                              */
+  float lr;              /* Adam learning rate                          */
+  float beta1, beta2, eps;
+  uint32_t seed;         /* minibatch sampling seed (nonzero)           */
+  int64_t holdout;       /* trailing samples kept out of training        */
+} mtc_mlp_train_desc;
+
+/* Held-out error of the network by predicted-value bin, and the fallback
+ * rule a learned frame applies: a step goes to the cone when its local
+ * density exceeds fallback_density or its prediction falls in a bin whose
+ * RMS error exceeds max_rms. */
+typedef struct mtc_mlp_calibration {
+  float lo, hi;             /* bin range of the predicted value          */
+  float rms[16];            /* held-out RMS error per bin                */
+  uint32_t count[16];       /* held-out samples per bin                  */
+  uint32_t mask;            /* bins sent to the cone                     */
+  float max_rms;
+  float fallback_density;
+  float rms_all;            /* held-out RMS error over all bins          */
+  float rms_mean_predictor; /* RMS of predicting the training mean       */
+} mtc_mlp_calibration;
+
+/* A network with room for `capacity` training samples on `device`.  The
+ * parameters start from a seeded fan-in-scaled uniform draw (seed 1). */
+int mtc_mlp_create(int32_t device, int64_t capacity, mtc_mlp** out);
+int mtc_mlp_destroy(mtc_mlp* m);
+/* Re-draw the parameters from `seed` and clear the optimiser state. */
+int mtc_mlp_init(mtc_mlp* m, uint32_t seed);
+int mtc_mlp_set_params(mtc_mlp* m, const float* params, size_t n);
+int mtc_mlp_get_params(mtc_mlp* m, float* params, size_t n);
+/* Append training samples: every pixel of ctx's view is marched with
+ * these uniforms (ctx's volumes; ctx must not be MTC_KERNEL_PLAIN) and
+ * each in-cloud step is kept with probability keep_fraction.  *n_total
+ * (may be NULL) receives the stored sample count; samples beyond the
+ * capacity are dropped. */
+int mtc_mlp_collect(mtc_mlp* m, mtc_ctx* ctx, const mtc_camera* cam, const mtc_time* time,
+                    const mtc_sun* sun, const mtc_cloud_params* params, uint32_t seed,
+                    float keep_fraction, int64_t* n_total);
+int mtc_mlp_sample_count(mtc_mlp* m, int64_t* n);
+int mtc_mlp_clear_samples(mtc_mlp* m);
+/* Replace the stored samples (xs: n * 9 floats, ys: n floats); their
+ * keys read back as UINT64_MAX. */
+int mtc_mlp_set_samples(mtc_mlp* m, const float* xs, const float* ys, int64_t n);
+/* Copy samples [first, first + n): inputs, targets and the key of each
+ * collected sample, py << 40 | px << 20 | step index.  Any output may be
+ * NULL. */
+int mtc_mlp_read_samples(mtc_mlp* m, int64_t first, int64_t n, float* xs, float* ys, uint64_t* keys);
+/* Minibatch Adam on the stored samples (all but the trailing holdout),
+ * loss = mean squared error.  loss_curve (may be NULL) receives the
+ * minibatch loss of every step. */
+int mtc_mlp_train(mtc_mlp* m, const mtc_mlp_train_desc* d, float* loss_curve);
+/* Gradient of mean((f(x) - y)^2) over n host samples (n <= 65536) at the
+ * current parameters, as the training step computes it; grad holds
+ * MTC_MLP_PARAMS floats. */
+int mtc_mlp_gradient(mtc_mlp* m, const float* xs, const float* ys, int32_t n, float* grad, float* loss);
+/* Network outputs for n host inputs, through the frame kernel's
+ * inference routine. */
+int mtc_mlp_predict(mtc_mlp* m, const float* xs, int32_t n, float* ys);
+/* Measure the held-out error and set the fallback rule. */
+int mtc_mlp_calibrate(mtc_mlp* m, int64_t holdout, float max_rms, float fallback_density,
+                      mtc_mlp_calibration* out);
+/* Attach a network to a MTC_KERNEL_LEARNED context (it must outlive its
+ * use there).  One in audit_period (power of two, 0 = none) of the steps
+ * the network answers is also cone-marched and its error counted
+ * (mtc_counters.mlp_audits / mlp_audit_sq_err). */
+int mtc_set_learned_light(mtc_ctx* ctx, mtc_mlp* m, int32_t audit_period);
 
 #ifdef __cplusplus
 }

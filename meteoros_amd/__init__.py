@@ -6,8 +6,8 @@ from meteoros_amd/csrc for gfx950.  This package only loads it and mirrors
 the reference's host objects (Camera, Scene time, Sky/Renderer) over it.
 """
 from ._abi import LIB_PATH, MtcError, lib  # noqa: F401
-from .stage import (Batch, Camera, CloudStage, Time, VolumeSpec, device_count, halton,  # noqa: F401
+from .stage import (Batch, Camera, CloudStage, LightNet, Time, VolumeSpec, device_count, halton,  # noqa: F401
                     preset_params, reference_params, sun_from_angles)
 
-__all__ = ["Batch", "Camera", "CloudStage", "Time", "VolumeSpec", "device_count", "halton", "preset_params",
+__all__ = ["Batch", "Camera", "CloudStage", "LightNet", "Time", "VolumeSpec", "device_count", "halton", "preset_params",
            "reference_params", "sun_from_angles", "lib", "LIB_PATH", "MtcError"]

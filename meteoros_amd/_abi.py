@@ -33,6 +33,8 @@ MTC_KERNEL_LEARNED = 3
 MTC_FLAG_COUNTERS = 1
 MTC_FLAG_FULL_MARCH = 2
 MTC_FLAG_TIMING = 4
+MTC_DISPLAY_RGBA8 = 0
+MTC_DISPLAY_RGBA32F = 1
 
 
 class MtcCamera(C.Structure):
@@ -93,7 +95,7 @@ class MtcDesc(C.Structure):
 class MtcCounters(C.Structure):
     _fields_ = [(n, C.c_uint64) for n in (
         "rays_marched", "rays_ocean", "rays_sky", "primary_steps", "cloud_steps", "cone_samples",
-        "cone_erosions", "early_exits", "mlp_queries", "mlp_fallbacks")]
+        "cone_erosions", "early_exits", "mlp_queries", "mlp_fallbacks", "mlp_audits", "mlp_audit_sq_err")]
 
 
 class MtcTimings(C.Structure):
@@ -103,6 +105,21 @@ class MtcTimings(C.Structure):
 class MtcBatchTuning(C.Structure):
     _fields_ = [("lanes_per_ray", C.c_int32), ("tile_rays_x", C.c_int32), ("blocks_per_cu", C.c_int32),
                 ("order", C.c_int32)]
+
+
+MTC_MLP_INPUTS = 9
+MTC_MLP_PARAMS = 9025
+
+
+class MtcMlpTrainDesc(C.Structure):
+    _fields_ = [("steps", C.c_int32), ("batch", C.c_int32), ("lr", C.c_float), ("beta1", C.c_float),
+                ("beta2", C.c_float), ("eps", C.c_float), ("seed", C.c_uint32), ("holdout", C.c_int64)]
+
+
+class MtcMlpCalibration(C.Structure):
+    _fields_ = [("lo", C.c_float), ("hi", C.c_float), ("rms", C.c_float * 16), ("count", C.c_uint32 * 16),
+                ("mask", C.c_uint32), ("max_rms", C.c_float), ("fallback_density", C.c_float),
+                ("rms_all", C.c_float), ("rms_mean_predictor", C.c_float)]
 
 
 P = C.POINTER
@@ -143,6 +160,9 @@ SIGNATURES = [
     ("mtc_reset_timings", C.c_int, [C.c_void_p]),
     ("mtc_reprojection_indices", C.c_int, [C.c_void_p, P(MtcCamera), P(MtcCamera), P(MtcTime), P(MtcCloudParams),
                                            C.c_void_p, C.c_size_t]),
+    ("mtc_god_rays", C.c_int, [C.c_void_p, P(MtcCamera), P(MtcSun), C.c_void_p]),
+    ("mtc_tone_map", C.c_int, [C.c_void_p, P(MtcTime), C.c_int32, C.c_void_p]),
+    ("mtc_read_display", C.c_int, [C.c_void_p, C.c_void_p, C.c_size_t]),
     ("mtc_batch_create", C.c_int, [C.c_void_p, C.c_int32, P(C.c_int32), P(C.c_int32), P(C.c_void_p)]),
     ("mtc_batch_destroy", C.c_int, [C.c_void_p]),
     ("mtc_batch_set_tuning", C.c_int, [C.c_void_p, P(MtcBatchTuning)]),
@@ -152,6 +172,22 @@ SIGNATURES = [
                                      P(MtcBatchTuning), P(C.c_double)]),
     ("mtc_batch_last_ms", C.c_int, [C.c_void_p, P(C.c_double)]),
     ("mtc_batch_read", C.c_int, [C.c_void_p, C.c_int32, C.c_void_p, C.c_void_p, C.c_size_t]),
+    ("mtc_mlp_create", C.c_int, [C.c_int32, C.c_int64, P(C.c_void_p)]),
+    ("mtc_mlp_destroy", C.c_int, [C.c_void_p]),
+    ("mtc_mlp_init", C.c_int, [C.c_void_p, C.c_uint32]),
+    ("mtc_mlp_set_params", C.c_int, [C.c_void_p, C.c_void_p, C.c_size_t]),
+    ("mtc_mlp_get_params", C.c_int, [C.c_void_p, C.c_void_p, C.c_size_t]),
+    ("mtc_mlp_collect", C.c_int, [C.c_void_p, C.c_void_p, P(MtcCamera), P(MtcTime), P(MtcSun), P(MtcCloudParams),
+                                  C.c_uint32, C.c_float, P(C.c_int64)]),
+    ("mtc_mlp_sample_count", C.c_int, [C.c_void_p, P(C.c_int64)]),
+    ("mtc_mlp_clear_samples", C.c_int, [C.c_void_p]),
+    ("mtc_mlp_set_samples", C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_int64]),
+    ("mtc_mlp_read_samples", C.c_int, [C.c_void_p, C.c_int64, C.c_int64, C.c_void_p, C.c_void_p, C.c_void_p]),
+    ("mtc_mlp_train", C.c_int, [C.c_void_p, P(MtcMlpTrainDesc), C.c_void_p]),
+    ("mtc_mlp_gradient", C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_int32, C.c_void_p, P(C.c_float)]),
+    ("mtc_mlp_predict", C.c_int, [C.c_void_p, C.c_void_p, C.c_int32, C.c_void_p]),
+    ("mtc_mlp_calibrate", C.c_int, [C.c_void_p, C.c_int64, C.c_float, C.c_float, P(MtcMlpCalibration)]),
+    ("mtc_set_learned_light", C.c_int, [C.c_void_p, C.c_void_p, C.c_int32]),
 ]
 
 _lib = None

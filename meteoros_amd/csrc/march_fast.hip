@@ -28,6 +28,7 @@
 //    volumes, the shell quadratic in FP64; transmittance and radiance stay
 //    FP32.
 #include "march_common.h"
+#include "mlp_common.h"
 
 typedef _Float16 h2v __attribute__((ext_vector_type(2)));
 
@@ -153,6 +154,13 @@ struct FastViews {
   const uint4* curl;
   const uint4* weather;
   int lf_sh, hf_sh, curl_sh, weather_sh;
+  // learned light (MTC_KERNEL_LEARNED): network and its error calibration
+  const bf16x8* mlp_frag;
+  const float* mlp_params;
+  float calib_lo, calib_inv;  // bin = (y - lo) * inv, 16 bins
+  unsigned int calib_mask;    // bins whose held-out error is too high
+  float fallback_density;     // local density above which the cone runs
+  int audit;                  // also cone-march one in `audit` answered steps (power of two, 0 = off)
 };
 
 // Shell entry/exit of a ray (raySphereIntersection, :229-273, with the
@@ -300,11 +308,90 @@ struct ConeLds {
 // :674-684 over the chunk in step order (identical values in all G lanes),
 // so the result equals the sequential march.  The loop is wave-uniform: a
 // lane whose ray has finished still takes its share of the cone work.
-template <int COV, bool F16, bool FP64SHELL, bool COUNT, int G>
+// Per-ray constants of rayMarch (:565-624) for a ray that is marched.
+template <bool FP64SHELL>
+__device__ __forceinline__ void setup_march(const FrameConsts& fc, const Ray& ray, int pixelID, RayMarch& rm) {
+  const float d = ray.d.y;
+    rm.ray = ray;
+    rm.c = mk(ray.o.x, -fc.earth_r, ray.o.z);
+    f3 startPos;
+    shell_span<FP64SHELL>(fc, ray, rm.c, rm.start_t, rm.end_t, startPos);
+    float maxSteps = (fc.step_mode == MTC_STEPS_FIXED)
+                         ? (float)fc.steps_max
+                         : floorf(mixf((float)fc.steps_min, (float)fc.steps_max, 1.0f - d));
+    rm.stepSize = (rm.end_t - rm.start_t) / maxSteps;
+    rm.cos_angle = dot(normalize(ray.d), ld3(fc.light_dir));
+    rm.HG_light = fmaxf(hg(rm.cos_angle, fc.hg_g), fc.hg_si * hg(rm.cos_angle, fc.hg_g2));
+    rm.inv_thick = 1.0f / fc.thickness;
+    rm.inv_div = 1.0f / fc.sample_divisor;
+    rm.ls = length(startPos - ray.o);
+    rm.inner = mk(rm.c.x, fc.inner_off, rm.c.z);
+    const float wsk = fc.top_offset * fc.wind_skew;
+    rm.wk = mk(fc.wind_dir[0] * wsk, fc.wind_dir[1] * wsk, fc.wind_dir[2] * wsk);
+    rm.woff = ld3(fc.wind_offset);
+    rm.ws = fc.weather_scale;
+    rm.cone_scale = rm.stepSize * rm.inv_thick;
+    rm.pixelID = pixelID;
+    rm.cap = 4 * max(fc.steps_min, fc.steps_max) + 64;
+}
+
+// The network's nine inputs for one in-cloud step (learned light):
+//   0-2  position in the shell, as the cone sees it: the cone samples the
+//        LF volume at rel + offsets (period 1) and the weather map at
+//        (rel.x, rel.z) * weather_scale, so x and z are taken modulo the
+//        weather tile, y modulo the LF tile
+//   3    height fraction
+//   4    local density (sampleLowFrequency, before base_density_factor)
+//   5-7  sun direction
+//   8    cos of the view ray's zenith angle: with the camera in the
+//        atmosphere it fixes the shell span, hence the primary step and the
+//        cone's length (step / thickness)
+__device__ __forceinline__ void mlp_features(const FrameConsts& fc, const Prim& pr, const RayMarch& rm, float f[9]) {
+  const float ax = pr.rel.x * rm.ws, az = pr.rel.z * rm.ws;
+  f[0] = ax - floorf(ax);
+  f[1] = pr.rel.y - floorf(pr.rel.y);
+  f[2] = az - floorf(az);
+  f[3] = pr.h;
+  f[4] = pr.base / fc.base_density_factor;
+  f[5] = fc.light_dir[0];
+  f[6] = fc.light_dir[1];
+  f[7] = fc.light_dir[2];
+  f[8] = rm.ray.d.y;
+}
+
+// Cone light march of one in-cloud step, one lane (GetLightEnergy's
+// optical depth, :338-373): the same samples and summation order as the
+// compacted march.
+template <int COV>
+__device__ __forceinline__ float cone_depth(const FrameConsts& fc, const FastViews& fv, const float* skern,
+                                           const Prim& pr, float cone_scale) {
+  float dal = 0.0f;
+  for (int i = 0; i < fc.light_samples; ++i) {
+    const float s = (float)i * cone_scale;
+    const f3 slp = mk(pr.rel.x + skern[3 * i] * s, pr.rel.y + skern[3 * i + 1] * s, pr.rel.z + skern[3 * i + 2] * s);
+    float2 clf = sample_lf<false>(fv.lf, fv.lf_sh, slp);
+    float2 cw = make_float2(0.0f, 0.0f);
+    if (COV == MTC_COVERAGE_WEATHER)
+      cw = sample_map<false>(fv.weather, fv.weather_sh, slp.x * fc.weather_scale, slp.z * fc.weather_scale);
+    const float cb = density<COV>(clf, cw, fc.coverage_const, pr.sc, pr.st);
+    dal += cb > 0.0f ? (fc.cone_erosion_scale * cb - pr.m) * pr.inv1m : 0.0f;
+  }
+  return dal;
+}
+
+__device__ __forceinline__ uint32_t step_hash(uint32_t a, uint32_t b, uint32_t c, uint32_t seed) {
+  uint32_t h = a * 0x9E3779B1u ^ b * 0x85EBCA77u ^ c * 0xC2B2AE3Du ^ seed * 0x27D4EB2Fu;
+  h ^= h >> 15;
+  h *= 0x2C1B3C6Du;
+  h ^= h >> 12;
+  return h;
+}
+
+template <int COV, bool F16, bool FP64SHELL, bool COUNT, int G, bool LEARN = false>
 __device__ __forceinline__ void march_tile(const FrameConsts& fc, const FastViews& fv, float4* __restrict__ out_color,
                                            float4* __restrict__ out_mask, unsigned long long* counters, int full,
                                            int rx0, int ry0, int tile_x, const float* sjit, const float* skern,
-                                           ConeLds* cone_lds) {
+                                           ConeLds* cone_lds, uint16_t* mlp_lds = nullptr) {
   const int lane = threadIdx.x & 63;
   const int wave = threadIdx.x >> 6;
   const int slot = threadIdx.x / G;  // ray slot in the tile
@@ -320,7 +407,8 @@ __device__ __forceinline__ void march_tile(const FrameConsts& fc, const FastView
   const int nls = fc.light_samples;
   ConeLds& L = cone_lds[wave];
   unsigned long long n_steps = 0, n_cloud = 0, n_cone = 0, n_cone_er = 0, n_early = 0;
-  unsigned long long n_ocean = 0, n_sky = 0, n_marched = 0;
+  unsigned long long n_ocean = 0, n_sky = 0, n_marched = 0, n_mlp = 0, n_mlp_fb = 0, n_audit = 0;
+  float audit_sq = 0.0f;
 
   // --- per-ray set-up (main, :690-757) --------------------------------------
   RayMarch rm;
@@ -347,27 +435,7 @@ __device__ __forceinline__ void march_tile(const FrameConsts& fc, const FastView
       } else {
         n_marched = 1;
         gdone = false;
-        rm.ray = ray;
-        rm.c = mk(ray.o.x, -fc.earth_r, ray.o.z);
-        f3 startPos;
-        shell_span<FP64SHELL>(fc, ray, rm.c, rm.start_t, rm.end_t, startPos);
-        float maxSteps = (fc.step_mode == MTC_STEPS_FIXED)
-                             ? (float)fc.steps_max
-                             : floorf(mixf((float)fc.steps_min, (float)fc.steps_max, 1.0f - d));
-        rm.stepSize = (rm.end_t - rm.start_t) / maxSteps;
-        rm.cos_angle = dot(normalize(ray.d), ld3(fc.light_dir));
-        rm.HG_light = fmaxf(hg(rm.cos_angle, fc.hg_g), fc.hg_si * hg(rm.cos_angle, fc.hg_g2));
-        rm.inv_thick = 1.0f / fc.thickness;
-        rm.inv_div = 1.0f / fc.sample_divisor;
-        rm.ls = length(startPos - ray.o);
-        rm.inner = mk(rm.c.x, fc.inner_off, rm.c.z);
-        const float wsk = fc.top_offset * fc.wind_skew;
-        rm.wk = mk(fc.wind_dir[0] * wsk, fc.wind_dir[1] * wsk, fc.wind_dir[2] * wsk);
-        rm.woff = ld3(fc.wind_offset);
-        rm.ws = fc.weather_scale;
-        rm.cone_scale = rm.stepSize * rm.inv_thick;
-        rm.pixelID = pixelID;
-        rm.cap = 4 * max(fc.steps_min, fc.steps_max) + 64;
+        setup_march<FP64SHELL>(fc, ray, pixelID, rm);
       }
     }
   }
@@ -384,11 +452,57 @@ __device__ __forceinline__ void march_tile(const FrameConsts& fc, const FastView
     // compacted cone march of this chunk's in-cloud steps
     float E = 0.0f;
     int cone_hits = 0;
-    const unsigned long long bal = __ballot(pr.cloud);
-    if (bal) {
+    float dal_mlp = 0.0f;
+    bool need_cone = pr.cloud;
+    bool audit = false;
+    if (LEARN) {
+      // every in-cloud step queries the network (one row per step, ranked
+      // by ballot); steps it should not be trusted with go to the cone
+      const unsigned long long qbal = __ballot(pr.cloud);
+      if (qbal) {
+        uint16_t* X = mlp_lds + wave * 64 * 16;
+        const int nq = __popcll(qbal);
+        const int qrank = __popcll(qbal & ((1ull << lane) - 1ull));
+        if (pr.cloud) {
+          float f[9];
+          mlp_features(fc, pr, rm, f);
+          uint32_t w[8];
+#pragma unroll
+          for (int k = 0; k < 8; ++k) {
+            const float lo = 2 * k < 9 ? f[2 * k] : 0.0f, hi = 2 * k + 1 < 9 ? f[2 * k + 1] : 0.0f;
+            w[k] = (uint32_t)mlp::f2bf(lo) | ((uint32_t)mlp::f2bf(hi) << 16);
+          }
+          uint4* row = (uint4*)(X + qrank * 16);
+          row[0] = make_uint4(w[0], w[1], w[2], w[3]);
+          row[1] = make_uint4(w[4], w[5], w[6], w[7]);
+        }
+        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+        __builtin_amdgcn_wave_barrier();
+        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+        const float y = mlp::infer_wave(X, fv.mlp_frag, fv.mlp_params, nq);
+        const float ym = __shfl(y, qrank, 64);
+        if (pr.cloud) {
+          int bin = (int)((ym - fv.calib_lo) * fv.calib_inv);
+          bin = bin < 0 ? 0 : (bin > 15 ? 15 : bin);
+          const bool fb = pr.base / fc.base_density_factor > fv.fallback_density || ((fv.calib_mask >> bin) & 1u);
+          if (!fb) {
+            dal_mlp = -ym;
+            need_cone = false;
+            audit = fv.audit && (step_hash((uint32_t)px, (uint32_t)py, (uint32_t)(k0 + j), 0u) & (uint32_t)(fv.audit - 1)) == 0u;
+          }
+        }
+        // the next chunk rewrites the query rows
+        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+        __builtin_amdgcn_wave_barrier();
+        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+      }
+    }
+    const bool run_cone = need_cone || audit;
+    const unsigned long long bal = __ballot(run_cone);
+    if (__ballot(pr.cloud)) {
       const int ncl = __popcll(bal);
       const int rank = __popcll(bal & ((1ull << lane) - 1ull));
-      if (pr.cloud) {
+      if (run_cone) {
         L.own[rank][0] = make_float4(pr.rel.x, pr.rel.y, pr.rel.z, rm.cone_scale);
         L.own[rank][1] = make_float4(pr.sc, pr.st, pr.m, pr.inv1m);
       }
@@ -413,11 +527,23 @@ __device__ __forceinline__ void march_tile(const FrameConsts& fc, const FastView
       __builtin_amdgcn_wave_barrier();
       __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
       if (pr.cloud) {
-        float dal = 0.0f;
-        for (int i = 0; i < nls; ++i) {
-          dal += L.contrib[rank * nls + i];
-          if (COUNT) cone_hits += L.hit[rank * nls + i];
+        float dal = dal_mlp;
+        if (run_cone) {
+          float cone = 0.0f;
+          for (int i = 0; i < nls; ++i) {
+            cone += L.contrib[rank * nls + i];
+            if (COUNT) cone_hits += L.hit[rank * nls + i];
+          }
+          if (need_cone) {
+            dal = cone;
+          } else if (COUNT) {
+            // the cone judges the network's answer for this step
+            const float err = dal_mlp - cone;
+            n_audit++;
+            audit_sq += err * err;
+          }
         }
+        if (COUNT && !need_cone) cone_hits = -1;  // marks a step answered by the network
         // GetLightEnergy (:331-388): exp(-dl) once for both identical terms
         const float h = pr.h;
         float prim = fast_exp(-dal);
@@ -449,8 +575,14 @@ __device__ __forceinline__ void march_tile(const FrameConsts& fc, const FastView
           if (cl) {
             if (COUNT && j == 0) {
               n_cloud++;
-              n_cone += (unsigned)nls;
-              n_cone_er += (unsigned)ch;
+              if (ch >= 0) {
+                n_cone += (unsigned)nls;
+                n_cone_er += (unsigned)ch;
+              }
+              if (LEARN) {
+                n_mlp++;
+                if (ch >= 0) n_mlp_fb++;
+              }
             }
             accum += da;
             transmittance = mixf(transmittance, e, 1.0f - accum);
@@ -492,6 +624,12 @@ __device__ __forceinline__ void march_tile(const FrameConsts& fc, const FastView
     wave_count(counters + 5, n_cone);
     wave_count(counters + 6, n_cone_er);
     wave_count(counters + 7, n_early);
+    if (LEARN) {
+      wave_count(counters + 8, n_mlp);
+      wave_count(counters + 9, n_mlp_fb);
+      wave_count(counters + 10, n_audit);
+      wave_count(counters + 11, (unsigned long long)rintf(audit_sq * 1.0e6f));
+    }
   }
 }
 
@@ -507,6 +645,69 @@ __global__ __launch_bounds__(256) void march_kernel(FrameConsts fc, FastViews fv
   constexpr int TX = 8, TY = 256 / G / 8;
   march_tile<COV, F16, FP64SHELL, COUNT, G>(fc, fv, ov.color, ov.mask, ov.counters, full, blockIdx.x * TX,
                                             blockIdx.y * TY, TX, sjit, skern, cone_lds);
+}
+
+// One frame with learned light (fast variant's sampling): the in-cloud
+// steps of each chunk query the network on MFMA before the cone march.
+template <int COV, bool COUNT, int G>
+__global__ __launch_bounds__(256) void march_learned_kernel(FrameConsts fc, FastViews fv, OutputViews ov, int full) {
+  __shared__ float sjit[48];
+  __shared__ float skern[18];
+  __shared__ ConeLds cone_lds[4];
+  __shared__ __align__(16) uint16_t mlp_lds[4 * 64 * 16];
+  if (threadIdx.x < 48) sjit[threadIdx.x] = (&fc.step_jitter[0][0])[threadIdx.x];
+  if (threadIdx.x < 18) skern[threadIdx.x] = (&fc.kern[0][0])[threadIdx.x];
+  __syncthreads();
+  constexpr int TX = 8, TY = 256 / G / 8;
+  march_tile<COV, false, true, COUNT, G, true>(fc, fv, ov.color, ov.mask, ov.counters, full, blockIdx.x * TX,
+                                               blockIdx.y * TY, TX, sjit, skern, cone_lds, mlp_lds);
+}
+
+// Training samples for the learned light: every pixel of the view is
+// marched (one lane per ray, the same steps and early exit as the frame
+// kernels); each in-cloud step is kept with probability keep / 2^32 and
+// written as (nine inputs, -cone optical depth, key = py << 40 | px << 20 |
+// step) at an atomically claimed slot.  Slots past `cap` are counted but
+// not written.
+template <int COV>
+__global__ __launch_bounds__(256) void light_samples_kernel(FrameConsts fc, FastViews fv, uint32_t seed, uint32_t keep,
+                                                            float* __restrict__ xs, float* __restrict__ ys,
+                                                            unsigned long long* __restrict__ keys,
+                                                            unsigned long long* __restrict__ count,
+                                                            unsigned long long cap) {
+  __shared__ float sjit[48];
+  __shared__ float skern[18];
+  if (threadIdx.x < 48) sjit[threadIdx.x] = (&fc.step_jitter[0][0])[threadIdx.x];
+  if (threadIdx.x < 18) skern[threadIdx.x] = (&fc.kern[0][0])[threadIdx.x];
+  __syncthreads();
+  const int px = blockIdx.x * 16 + (threadIdx.x & 15);
+  const int py = blockIdx.y * 16 + (threadIdx.x >> 4);
+  if (px >= fc.width || py >= fc.height) return;
+  const float ux = (float)px / (float)fc.width, uy = 1.0f - (float)py / (float)fc.height;
+  const Ray ray = cast_ray(fc, fc.view, ux, uy, fc.jit_march[0], fc.jit_march[1]);
+  if (ray.d.y < 0.0f || ray.d.y < fc.fade_start) return;
+  RayMarch rm;
+  setup_march<true>(fc, ray, fc.frame, rm);
+  float accum = 0.0f;
+  for (int k = 0;; ++k) {
+    const Prim pr = eval_primary<COV, false>(fc, fv, sjit, rm, k);
+    if (!pr.valid) break;
+    if (pr.cloud) {
+      accum += pr.da;
+      if (step_hash((uint32_t)px, (uint32_t)py, (uint32_t)k, seed) < keep) {
+        float f[9];
+        mlp_features(fc, pr, rm, f);
+        const float dal = cone_depth<COV>(fc, fv, skern, pr, rm.cone_scale);
+        const unsigned long long slot = atomicAdd(count, 1ull);
+        if (slot < cap) {
+          for (int i = 0; i < 9; ++i) xs[slot * 9 + i] = f[i];
+          ys[slot] = -dal;
+          keys[slot] = ((unsigned long long)py << 40) | ((unsigned long long)px << 20) | (unsigned long long)k;
+        }
+      }
+    }
+    if (accum >= 1.0f) break;
+  }
 }
 
 // Many views in one launch: a persistent grid pulls (view, tile) work items
@@ -674,6 +875,65 @@ hipError_t launch_march_fast(const FrameConsts& fc, const uint4* lf, const uint4
   bool ok = variant == 0 ? launch_g<false, true>(lanes_per_ray, stream, fc, fv, ov, full)
                          : launch_g<true, true>(lanes_per_ray, stream, fc, fv, ov, full);
   if (!ok) return hipErrorInvalidValue;
+  return hipGetLastError();
+}
+
+static mtc::fast::FastViews learned_views(const uint4* lf, const uint4* hf, const uint4* curl, const uint4* weather,
+                                          const VolumeViews& vv, const LearnedLight& ll) {
+  mtc::fast::FastViews fv{lf, hf, curl, weather, vv.lf_shift, vv.hf_shift, vv.curl_shift, vv.weather_shift};
+  fv.mlp_frag = (const bf16x8*)ll.frag;
+  fv.mlp_params = ll.params;
+  fv.calib_lo = ll.calib_lo;
+  fv.calib_inv = ll.calib_inv;
+  fv.calib_mask = ll.calib_mask;
+  fv.fallback_density = ll.fallback_density;
+  fv.audit = ll.audit;
+  return fv;
+}
+
+template <int COV, int G>
+static void launch_learned_v(hipStream_t s, const FrameConsts& fc, const mtc::fast::FastViews& fv,
+                             const OutputViews& ov, int full) {
+  int rays_x = full ? fc.width : (fc.width + 3) / 4;
+  int rays_y = full ? fc.height : (fc.height + 3) / 4;
+  constexpr int TX = 8, TY = 256 / G / 8;
+  dim3 grid((rays_x + TX - 1) / TX, (rays_y + TY - 1) / TY), block(256);
+  if (ov.counters)
+    hipLaunchKernelGGL((mtc::fast::march_learned_kernel<COV, true, G>), grid, block, 0, s, fc, fv, ov, full);
+  else
+    hipLaunchKernelGGL((mtc::fast::march_learned_kernel<COV, false, G>), grid, block, 0, s, fc, fv, ov, full);
+}
+
+// Learned-light frame (fast sampling, FP64 shell); lanes_per_ray 4 or 8.
+hipError_t launch_march_learned(const FrameConsts& fc, const uint4* lf, const uint4* hf, const uint4* curl,
+                                const uint4* weather, const VolumeViews& vv, const OutputViews& ov, int full,
+                                int lanes_per_ray, const LearnedLight& ll, hipStream_t s) {
+  const mtc::fast::FastViews fv = learned_views(lf, hf, curl, weather, vv, ll);
+  const bool wx = fc.coverage_mode == MTC_COVERAGE_WEATHER;
+  if (lanes_per_ray == 8) {
+    if (wx) launch_learned_v<MTC_COVERAGE_WEATHER, 8>(s, fc, fv, ov, full);
+    else launch_learned_v<MTC_COVERAGE_CONSTANT, 8>(s, fc, fv, ov, full);
+  } else if (lanes_per_ray == 4) {
+    if (wx) launch_learned_v<MTC_COVERAGE_WEATHER, 4>(s, fc, fv, ov, full);
+    else launch_learned_v<MTC_COVERAGE_CONSTANT, 4>(s, fc, fv, ov, full);
+  } else {
+    return hipErrorInvalidValue;
+  }
+  return hipGetLastError();
+}
+
+hipError_t launch_light_samples(const FrameConsts& fc, const uint4* lf, const uint4* hf, const uint4* curl,
+                                const uint4* weather, const VolumeViews& vv, uint32_t seed, uint32_t keep, float* xs,
+                                float* ys, unsigned long long* keys, unsigned long long* count, unsigned long long cap,
+                                hipStream_t s) {
+  mtc::fast::FastViews fv{lf, hf, curl, weather, vv.lf_shift, vv.hf_shift, vv.curl_shift, vv.weather_shift};
+  dim3 grid((fc.width + 15) / 16, (fc.height + 15) / 16), block(256);
+  if (fc.coverage_mode == MTC_COVERAGE_WEATHER)
+    hipLaunchKernelGGL((mtc::fast::light_samples_kernel<MTC_COVERAGE_WEATHER>), grid, block, 0, s, fc, fv, seed, keep,
+                       xs, ys, keys, count, cap);
+  else
+    hipLaunchKernelGGL((mtc::fast::light_samples_kernel<MTC_COVERAGE_CONSTANT>), grid, block, 0, s, fc, fv, seed,
+                       keep, xs, ys, keys, count, cap);
   return hipGetLastError();
 }
 

@@ -29,6 +29,9 @@ hipError_t launch_march_plain(const FrameConsts& fc, const VolumeViews& vv, cons
 hipError_t launch_march_fast(const FrameConsts& fc, const uint4* lf, const uint4* hf, const uint4* curl,
                              const uint4* weather, const VolumeViews& vv, const OutputViews& ov, int full, int variant,
                              int lanes_per_ray, hipStream_t stream);
+hipError_t launch_march_learned(const FrameConsts& fc, const uint4* lf, const uint4* hf, const uint4* curl,
+                                const uint4* weather, const VolumeViews& vv, const OutputViews& ov, int full,
+                                int lanes_per_ray, const LearnedLight& ll, hipStream_t s);
 hipError_t pack_fast_resources(const VolumeViews& vv, uint4* lf, uint4* hf, uint4* curl, uint4* weather,
                                hipStream_t s);
 hipError_t launch_reproject(const FrameConsts& fc, const OutputViews& ov, int mode, hipStream_t stream);
@@ -114,6 +117,8 @@ void release(mtc_ctx* c) {
   (void)hipFree(c->q_hf);
   (void)hipFree(c->q_curl);
   (void)hipFree(c->q_weather);
+  (void)hipFree(c->ray_fp);
+  (void)hipFree(c->display);
   if (c->stream) (void)hipStreamDestroy(c->stream);
   delete c;
 }
@@ -139,6 +144,13 @@ int launch_march(mtc_ctx* c, const FrameConsts& fc, const OutputViews& ov, int f
     case MTC_KERNEL_PRECISION:
       e = launch_march_fast(fc, c->q_lf, c->q_hf, c->q_curl, c->q_weather, vv, ov, full, 1, c->lanes_per_ray, s);
       break;
+    case MTC_KERNEL_LEARNED: {
+      LearnedLight ll;
+      int rc = learned_light_of(c, &ll);
+      if (rc) return rc;
+      e = launch_march_learned(fc, c->q_lf, c->q_hf, c->q_curl, c->q_weather, vv, ov, full, c->lanes_per_ray, ll, s);
+      break;
+    }
     default: return mtc_fail(MTC_ERR_UNSUPPORTED, "march: kernel variant not available in this build");
   }
   HIP_TRY(e, "march kernel launch");
@@ -193,7 +205,8 @@ int mtc_create(const mtc_desc* d, mtc_ctx** out) {
     return mtc_fail(MTC_ERR_INVALID_ARG, "mtc_create: unknown volume format");
   if (d->kernel < MTC_KERNEL_PLAIN || d->kernel > MTC_KERNEL_LEARNED)
     return mtc_fail(MTC_ERR_INVALID_ARG, "mtc_create: unknown kernel variant");
-  if (d->kernel == MTC_KERNEL_LEARNED) return mtc_fail(MTC_ERR_UNSUPPORTED, "mtc_create: learned-light kernel not built");
+  if (d->kernel == MTC_KERNEL_LEARNED && d->lanes_per_ray != 0 && d->lanes_per_ray != 4 && d->lanes_per_ray != 8)
+    return mtc_fail(MTC_ERR_INVALID_ARG, "mtc_create: the learned-light kernel takes lanes_per_ray 0, 4 or 8");
   if (d->lanes_per_ray != 0 && d->lanes_per_ray != 1 && d->lanes_per_ray != 2 && d->lanes_per_ray != 4 &&
       d->lanes_per_ray != 8)
     return mtc_fail(MTC_ERR_INVALID_ARG, "mtc_create: lanes_per_ray must be 0, 1, 2, 4 or 8");
@@ -425,6 +438,8 @@ int mtc_get_counters(mtc_ctx* c, mtc_counters* out) {
   out->early_exits = h[7];
   out->mlp_queries = h[8];
   out->mlp_fallbacks = h[9];
+  out->mlp_audits = h[10];
+  out->mlp_audit_sq_err = h[11];
   return MTC_OK;
 }
 

@@ -82,8 +82,27 @@ struct OutputViews {
   float4* color;      // current result image, W*H RGBA32F
   const float4* prev; // previous result image
   float4* mask;       // god-ray mask, W*H RGBA32F
-  unsigned long long* counters;  // 10 x u64 or nullptr
+  unsigned long long* counters;  // 12 x u64 or nullptr
   int32_t* reproj_idx;           // optional tap-index map
+};
+
+// Per-frame constants of the god-ray pass (postProcess_GodRays.frag:72-98),
+// folded on the host.
+struct PostConsts {
+  float sun_ss[2];  // clamp((proj * view * (0, 1, 0, 1)).xy + 1) / 2, 0, 1)
+  float blend;      // dot(normalize((0, 1, 0) - eye), camera forward)
+  float light[3];   // sunAndSky.lightColor.rgb
+  int32_t width, height;
+};
+
+// Learned-light network as a frame kernel sees it.
+struct LearnedLight {
+  const void* frag;       // packed BF16 MFMA weight fragments
+  const float* params;    // FP32 parameter vector (biases, output layer)
+  float calib_lo, calib_inv;
+  unsigned int calib_mask;
+  float fallback_density;
+  int audit;
 };
 
 // The context behind the opaque mtc_ctx handle.
@@ -110,8 +129,20 @@ struct mtc_ctx {
   std::vector<std::array<hipEvent_t, 3>> pending;
   std::vector<hipEvent_t> event_pool;
   mtc_timings totals = {0, 0.0, 0.0};
+  // post-process: decoded god-ray mask footprints ((W+1) x (H+1) float4)
+  // and the display image (RGBA8 or RGBA32F), allocated on first use
+  float4* ray_fp = nullptr;
+  void* display = nullptr;
+  int display_format = -1;
+  // MTC_KERNEL_LEARNED: attached network (not owned) and its audit period
+  mtc_mlp* mlp = nullptr;
+  int mlp_audit = 0;
   size_t lf_bytes = 0, hf_bytes = 0, curl_bytes = 0, weather_bytes = 0;
 };
 
 // Device views of a context's base resources.
 VolumeViews volume_views(const mtc_ctx* c);
+
+// The attached network's device views (mlp_api.cpp); MTC_ERR_STATE when
+// none is attached.
+int learned_light_of(const mtc_ctx* c, LearnedLight* out);

@@ -211,6 +211,21 @@ class CloudStage:
     def reset_timings(self) -> None:
         check(lib().mtc_reset_timings(self._ctx), "mtc_reset_timings")
 
+    # -- post-process passes (postProcess_GodRays.frag, postProcess_ToneMap.frag)
+    def god_rays(self, cam: A.MtcCamera, sun: A.MtcSun, stream: int | None = None) -> None:
+        check(lib().mtc_god_rays(self._ctx, C.byref(cam), C.byref(sun), stream), "mtc_god_rays")
+
+    def tone_map(self, time: A.MtcTime, as_float: bool = False, stream: int | None = None) -> None:
+        fmt = A.MTC_DISPLAY_RGBA32F if as_float else A.MTC_DISPLAY_RGBA8
+        check(lib().mtc_tone_map(self._ctx, C.byref(time), fmt, stream), "mtc_tone_map")
+        self._display_float = as_float
+
+    def read_display(self) -> np.ndarray:
+        as_float = getattr(self, "_display_float", False)
+        out = np.empty((self.height, self.width, 4), dtype=np.float32 if as_float else np.uint8)
+        check(lib().mtc_read_display(self._ctx, out.ctypes.data, out.nbytes), "mtc_read_display")
+        return out
+
     def reprojection_indices(self, cam, old, time, params) -> np.ndarray:
         n = self.width * self.height * int(params.motion_blur_samples)
         idx = np.empty(n, dtype=np.int32)
@@ -286,3 +301,117 @@ class Batch:
         check(lib().mtc_batch_read(self._b, int(view), rgba.ctypes.data, mask.ctypes.data, rgba.size),
               "mtc_batch_read")
         return rgba.reshape(h, w, 4), mask.reshape(h, w, 4)
+
+
+class LightNet:
+    """The learned-light network (9 -> 64 -> 64 -> 64 -> 1, SiLU) with its
+    device store of training samples.  No reference counterpart: it stands
+    in for the cone march's optical depth (cloudRayMarch.comp:338-373).
+    Training, inference and sample generation run on the GPU."""
+
+    N_PARAMS = A.MTC_MLP_PARAMS
+    N_INPUTS = A.MTC_MLP_INPUTS
+
+    def __init__(self, capacity: int = 1 << 20, device: int = 0):
+        self.capacity = int(capacity)
+        self._m = C.c_void_p()
+        check(lib().mtc_mlp_create(int(device), self.capacity, C.byref(self._m)), "mtc_mlp_create")
+
+    def close(self) -> None:
+        if self._m:
+            lib().mtc_mlp_destroy(self._m)
+            self._m = C.c_void_p()
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    @property
+    def handle(self) -> C.c_void_p:
+        return self._m
+
+    def init(self, seed: int = 1) -> None:
+        check(lib().mtc_mlp_init(self._m, int(seed)), "mtc_mlp_init")
+
+    @property
+    def params(self) -> np.ndarray:
+        p = np.empty(self.N_PARAMS, dtype=np.float32)
+        check(lib().mtc_mlp_get_params(self._m, p.ctypes.data, p.size), "mtc_mlp_get_params")
+        return p
+
+    @params.setter
+    def params(self, p: np.ndarray) -> None:
+        a = np.ascontiguousarray(p, dtype=np.float32).reshape(-1)
+        check(lib().mtc_mlp_set_params(self._m, a.ctypes.data, a.size), "mtc_mlp_set_params")
+
+    def collect(self, stage: CloudStage, cam, time, sun, params, seed: int, keep_fraction: float) -> int:
+        n = C.c_int64()
+        check(lib().mtc_mlp_collect(self._m, stage.handle, C.byref(cam), C.byref(time), C.byref(sun), C.byref(params),
+                                    int(seed), float(keep_fraction), C.byref(n)), "mtc_mlp_collect")
+        return int(n.value)
+
+    def sample_count(self) -> int:
+        n = C.c_int64()
+        check(lib().mtc_mlp_sample_count(self._m, C.byref(n)), "mtc_mlp_sample_count")
+        return int(n.value)
+
+    def clear_samples(self) -> None:
+        check(lib().mtc_mlp_clear_samples(self._m), "mtc_mlp_clear_samples")
+
+    def set_samples(self, xs: np.ndarray, ys: np.ndarray) -> None:
+        x = np.ascontiguousarray(xs, dtype=np.float32).reshape(-1, self.N_INPUTS)
+        y = np.ascontiguousarray(ys, dtype=np.float32).reshape(-1)
+        check(lib().mtc_mlp_set_samples(self._m, x.ctypes.data, y.ctypes.data, y.size), "mtc_mlp_set_samples")
+
+    def read_samples(self, first: int = 0, n: int | None = None):
+        if n is None:
+            n = self.sample_count() - first
+        xs = np.empty((n, self.N_INPUTS), dtype=np.float32)
+        ys = np.empty(n, dtype=np.float32)
+        keys = np.empty(n, dtype=np.uint64)
+        check(lib().mtc_mlp_read_samples(self._m, int(first), int(n), xs.ctypes.data, ys.ctypes.data,
+                                         keys.ctypes.data), "mtc_mlp_read_samples")
+        return xs, ys, keys
+
+    def train(self, steps: int, batch: int = 4096, lr: float = 2e-3, beta1: float = 0.9, beta2: float = 0.999,
+              eps: float = 1e-8, seed: int = 1, holdout: int = 0) -> np.ndarray:
+        d = A.MtcMlpTrainDesc(int(steps), int(batch), float(lr), float(beta1), float(beta2), float(eps), int(seed),
+                              int(holdout))
+        curve = np.empty(int(steps), dtype=np.float32)
+        check(lib().mtc_mlp_train(self._m, C.byref(d), curve.ctypes.data), "mtc_mlp_train")
+        return curve
+
+    def gradient(self, xs: np.ndarray, ys: np.ndarray):
+        x = np.ascontiguousarray(xs, dtype=np.float32).reshape(-1, self.N_INPUTS)
+        y = np.ascontiguousarray(ys, dtype=np.float32).reshape(-1)
+        g = np.empty(self.N_PARAMS, dtype=np.float32)
+        loss = C.c_float()
+        check(lib().mtc_mlp_gradient(self._m, x.ctypes.data, y.ctypes.data, y.size, g.ctypes.data, C.byref(loss)),
+              "mtc_mlp_gradient")
+        return g, float(loss.value)
+
+    def predict(self, xs: np.ndarray) -> np.ndarray:
+        x = np.ascontiguousarray(xs, dtype=np.float32).reshape(-1, self.N_INPUTS)
+        y = np.empty(x.shape[0], dtype=np.float32)
+        check(lib().mtc_mlp_predict(self._m, x.ctypes.data, x.shape[0], y.ctypes.data), "mtc_mlp_predict")
+        return y
+
+    def calibrate(self, holdout: int, max_rms: float, fallback_density: float = 0.8) -> dict:
+        c = A.MtcMlpCalibration()
+        check(lib().mtc_mlp_calibrate(self._m, int(holdout), float(max_rms), float(fallback_density), C.byref(c)),
+              "mtc_mlp_calibrate")
+        return {"lo": c.lo, "hi": c.hi, "rms": list(c.rms), "count": list(c.count), "mask": int(c.mask),
+                "max_rms": c.max_rms, "fallback_density": c.fallback_density, "rms_all": c.rms_all,
+                "rms_mean_predictor": c.rms_mean_predictor}
+
+    def attach(self, stage: CloudStage, audit_period: int = 0) -> None:
+        """Use this network in a MTC_KERNEL_LEARNED stage's frames."""
+        check(lib().mtc_set_learned_light(stage.handle, self._m, int(audit_period)), "mtc_set_learned_light")

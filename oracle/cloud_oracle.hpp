@@ -31,6 +31,7 @@
 #include <cmath>
 #include <cstdint>
 #include <cstring>
+#include <vector>
 
 #include "../include/meteoros_cloud.h"
 
@@ -445,10 +446,20 @@ static inline T erode(const Frame<T>& f, const Volumes& vol, T base, v3<T> p, T 
   return remap(base, mod * T(0.005), T(1.0), T(0), T(1));
 }
 
+// One in-cloud step as the learned light sees it: its step index, the nine
+// network inputs and the cone's log transmittance -dal (the training
+// target).  No GLSL counterpart; recorded by ray_march when asked.
+struct LightRecord {
+  int k;
+  double x[9];
+  double y;
+};
+
 // rayMarch, :565-688
 template <class T>
 static v3<T> ray_march(const Frame<T>& f, const Volumes& vol, const Ray<T>& ray, v3<T> c, v3<T> startPos,
-                       T start_t, T end_t, int pixelID, T& accum, Counters& cnt) {
+                       T start_t, T end_t, int pixelID, T& accum, Counters& cnt,
+                       std::vector<LightRecord>* rec = nullptr) {
   T d = dot(ray.d, V(T(0), T(1), T(0)));
   T maxSteps;
   if (f.step_mode == MTC_STEPS_FIXED) maxSteps = (T)f.steps_max;
@@ -508,6 +519,25 @@ static v3<T> ray_march(const Frame<T>& f, const Volumes& vol, const Ray<T>& ray,
           dal += erode(f, vol, f.cone_erosion_scale * cb, sk, h);
         }
       }
+      if (rec) {
+        // inputs: position in the shell (x, z modulo the weather tile, y
+        // modulo the LF tile), height, local density, sun direction, cos of
+        // the ray's zenith angle
+        v3<T> rel = rel_pos(f, pos, c);
+        LightRecord r;
+        r.k = iter;
+        r.x[0] = (double)fractT(rel.x * f.weather_scale);
+        r.x[1] = (double)fractT(rel.y);
+        r.x[2] = (double)fractT(rel.z * f.weather_scale);
+        r.x[3] = (double)h;
+        r.x[4] = (double)(base / f.base_density_factor);
+        r.x[5] = (double)lightDir.x;
+        r.x[6] = (double)lightDir.y;
+        r.x[7] = (double)lightDir.z;
+        r.x[8] = (double)ray.d.y;
+        r.y = -(double)dal;
+        rec->push_back(r);
+      }
       T E = light_energy(h, dal, base, HG_light, cos_angle, stepSize, f.brightness);
       transmittance = mixT(transmittance, E, T(1) - accum);
       ret = ret + V(transmittance, transmittance, transmittance);
@@ -524,7 +554,7 @@ static v3<T> ray_march(const Frame<T>& f, const Volumes& vol, const Ray<T>& ray,
 // main of cloudRayMarch.comp, :690-826, for one chosen pixel.
 template <class T>
 static void march_pixel(const Frame<T>& f, const Volumes& vol, int px, int py, int pixelID, T rgba[4], T mask[4],
-                        Counters& cnt) {
+                        Counters& cnt, std::vector<LightRecord>* rec = nullptr) {
   T ux = (T)px / (T)f.W, uy = (T)py / (T)f.H;
   uy = T(1) - uy;
   v3<T> eye = V(-f.eye_ubo[0], -f.eye_ubo[1], -f.eye_ubo[2]);
@@ -556,7 +586,7 @@ static void march_pixel(const Frame<T>& f, const Volumes& vol, int px, int py, i
   Isect<T> in = ray_sphere(ray.o, ray.d, c, f.r_in);
   Isect<T> out = ray_sphere(ray.o, ray.d, c, f.r_out);
   T accum = T(0);
-  v3<T> col = ray_march(f, vol, ray, c, in.point, in.t, out.t, pixelID, accum, cnt);
+  v3<T> col = ray_march(f, vol, ray, c, in.point, in.t, out.t, pixelID, accum, cnt, rec);
   T godAccum = accum;
   accum = accum * smoothstepT(T(0), T(1), std::fmin(T(1), remap(ray.d.y, f.fade_start, f.fade_end, T(0), T(1))));
   rgba[0] = mixT(bg.x, col.x, accum);

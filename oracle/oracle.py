@@ -38,6 +38,18 @@ def lib() -> C.CDLL:
         h.orc_march_pixels.argtypes = [vp, vp, vp, vp, P(A.MtcVolumeDesc), P(A.MtcCamera), P(A.MtcTime), P(A.MtcSun),
                                        P(A.MtcCloudParams), C.c_int, C.c_int, vp, C.c_int64, C.c_int, vp, vp, vp, vp,
                                        C.c_int]
+        h.orc_light_samples.restype = C.c_int64
+        h.orc_light_samples.argtypes = [vp, vp, vp, vp, P(A.MtcVolumeDesc), P(A.MtcCamera), P(A.MtcTime),
+                                        P(A.MtcSun), P(A.MtcCloudParams), C.c_int, C.c_int, vp, C.c_int64, C.c_int,
+                                        C.c_int64, vp, vp, vp, vp]
+        h.orc_god_rays.restype = C.c_int
+        h.orc_god_rays.argtypes = [P(A.MtcCamera), P(A.MtcSun), C.c_int, C.c_int, vp, vp, vp, C.c_int, vp]
+        h.orc_tone_map.restype = C.c_int
+        h.orc_tone_map.argtypes = [C.c_uint32, C.c_int, C.c_int, vp, vp, vp, C.c_int]
+        h.orc_wang_hash.restype = C.c_uint32
+        h.orc_wang_hash.argtypes = [C.c_uint32, C.c_uint32, C.c_uint32]
+        h.orc_uncharted2.restype = C.c_float
+        h.orc_uncharted2.argtypes = [C.c_float]
         h.orc_select_pixels.restype = C.c_int64
         h.orc_select_pixels.argtypes = [C.c_int, C.c_int, C.c_int, vp]
         h.orc_frame.restype = C.c_int
@@ -135,6 +147,61 @@ def march_pixels(vol: Volumes, cam, time, sun, params, W, H, xy: np.ndarray, pre
     if rc:
         raise ValueError("orc_march_pixels rejected its arguments")
     return rgba, mask, steps, dict(zip(COUNTER_NAMES, (int(c) for c in cnt)))
+
+
+def light_samples(vol: Volumes, cam, time, sun, params, W, H, xy: np.ndarray, precision=64):
+    """Learned-light training samples of the listed pixels (every in-cloud
+    step).  Returns (pixel index into xy [m], step index [m], inputs [m, 9],
+    target [m]) with target = -(cone optical depth)."""
+    xy = np.ascontiguousarray(xy, dtype=np.int32).reshape(-1, 2)
+    n = xy.shape[0]
+    cap = max(1, n * 64)
+    while True:
+        pix = np.empty(cap, dtype=np.int32)
+        step = np.empty(cap, dtype=np.int32)
+        x = np.empty((cap, 9), dtype=np.float64)
+        y = np.empty(cap, dtype=np.float64)
+        m = lib().orc_light_samples(*vol.ptrs(), C.byref(vol.desc), C.byref(cam), C.byref(time), C.byref(sun),
+                                    C.byref(params), W, H, xy.ctypes.data, n, precision, cap, pix.ctypes.data,
+                                    step.ctypes.data, x.ctypes.data, y.ctypes.data)
+        if m < 0:
+            raise ValueError("orc_light_samples rejected its arguments")
+        if m <= cap:
+            return pix[:m], step[:m], x[:m], y[:m]
+        cap = int(m)
+
+
+def god_rays(cam, sun, rgba: np.ndarray, mask: np.ndarray, precision=32):
+    """postProcess_GodRays.frag over an HxWx4 image and its god-ray mask.
+    Returns (image, (sun_ss_x, sun_ss_y, blend))."""
+    H, W = rgba.shape[:2]
+    a = np.ascontiguousarray(rgba, dtype=np.float32)
+    m = np.ascontiguousarray(mask, dtype=np.float32)
+    out = np.empty_like(a)
+    consts = np.zeros(3, dtype=np.float64)
+    if lib().orc_god_rays(C.byref(cam), C.byref(sun), W, H, a.ctypes.data, m.ctypes.data, out.ctypes.data, precision,
+                          consts.ctypes.data):
+        raise ValueError("orc_god_rays rejected its arguments")
+    return out, tuple(float(c) for c in consts)
+
+
+def tone_map(time_s: int, rgba: np.ndarray, precision=32):
+    """postProcess_ToneMap.frag: (rgb before quantisation [H,W,3], RGBA8 [H,W,4])."""
+    H, W = rgba.shape[:2]
+    a = np.ascontiguousarray(rgba, dtype=np.float32)
+    rgb = np.empty((H, W, 3), dtype=np.float64)
+    out8 = np.empty((H, W, 4), dtype=np.uint8)
+    if lib().orc_tone_map(int(time_s), W, H, a.ctypes.data, rgb.ctypes.data, out8.ctypes.data, precision):
+        raise ValueError("orc_tone_map rejected its arguments")
+    return rgb, out8
+
+
+def wang_hash(u: int, v: int, s: int) -> int:
+    return int(lib().orc_wang_hash(u, v, s))
+
+
+def uncharted2(x: float) -> float:
+    return float(lib().orc_uncharted2(x))
 
 
 def frame(vol: Volumes, cam, old, time, sun, params, W, H, prev=None, precision=32, full=False, threads=0,

@@ -10,11 +10,13 @@
 #include <vector>
 
 #include "cloud_oracle.hpp"
+#include "post_oracle.hpp"
 #include "noise_oracle.hpp"
 
 using orc::Counters;
 using orc::Frame;
 using orc::Volumes;
+using orc::LightRecord;
 
 static Volumes make_volumes(const void* lf, const void* hf, const uint8_t* curl, const uint8_t* weather,
                             const mtc_volume_desc* vd) {
@@ -80,6 +82,45 @@ int orc_march_pixels(const void* lf, const void* hf, const uint8_t* curl, const 
     march_list<float>(f, vol, xy, n, pid, rgba, mask, steps, counters, threads);
   }
   return 0;
+}
+
+// The learned light's training samples of the listed pixels, marched in
+// full (every pixel's own ray, pixelID = frame_count_mod16 as in the march):
+// for every in-cloud step its pixel, step index, nine network inputs and the
+// cone's log transmittance.  Writes up to `cap` records, returns how many
+// there are.  Single-threaded (small pixel lists).
+int64_t orc_light_samples(const void* lf, const void* hf, const uint8_t* curl, const uint8_t* weather,
+                          const mtc_volume_desc* vd, const mtc_camera* cam, const mtc_time* tm, const mtc_sun* sun,
+                          const mtc_cloud_params* p, int W, int H, const int32_t* xy, int64_t n, int precision,
+                          int64_t cap, int32_t* pix, int32_t* step, double* x, double* y) {
+  if (!lf || !hf || !curl || !weather || !vd || !cam || !tm || !sun || !p || !xy || W <= 0 || H <= 0) return -1;
+  Volumes vol = make_volumes(lf, hf, curl, weather, vd);
+  const int pid = tm->frame_count_mod16;
+  int64_t total = 0;
+  std::vector<LightRecord> rec;
+  for (int64_t i = 0; i < n; ++i) {
+    rec.clear();
+    Counters cnt;
+    if (precision == 64) {
+      Frame<double> f = orc::make_frame<double>(*cam, *tm, *sun, *p, W, H);
+      double rgba[4], mask[4];
+      orc::march_pixel<double>(f, vol, xy[2 * i], xy[2 * i + 1], pid, rgba, mask, cnt, &rec);
+    } else {
+      Frame<float> f = orc::make_frame<float>(*cam, *tm, *sun, *p, W, H);
+      float rgba[4], mask[4];
+      orc::march_pixel<float>(f, vol, xy[2 * i], xy[2 * i + 1], pid, rgba, mask, cnt, &rec);
+    }
+    for (const LightRecord& r : rec) {
+      if (total < cap) {
+        pix[total] = (int32_t)i;
+        step[total] = r.k;
+        for (int j = 0; j < 9; ++j) x[total * 9 + j] = r.x[j];
+        y[total] = r.y;
+      }
+      ++total;
+    }
+  }
+  return total;
 }
 
 // The 4x4 selection of cloudRayMarch.comp:695-704 for one frame: writes the
@@ -290,5 +331,67 @@ int orc_noise_generate(int kind, int N, int fmt, uint32_t seed, int z0, int nz, 
   return 0;
 }
 uint16_t orc_to_half(float f) { return orc_noise::to_half(f); }
+
+}  // extern "C"
+
+// ---------------------------------------------------------------------------
+// post-process passes (post_oracle.hpp)
+// ---------------------------------------------------------------------------
+extern "C" {
+
+// God rays of postProcess_GodRays.frag over a W x H RGBA32F image and its
+// RGBA32F god-ray mask; out receives the updated image, consts (may be
+// NULL) the folded sunPos_ss.xy and blend factor.
+int orc_god_rays(const mtc_camera* cam, const mtc_sun* sun, int W, int H, const float* rgba, const float* mask,
+                 float* out, int precision, double* consts) {
+  if (!cam || !sun || !rgba || !mask || !out || W <= 0 || H <= 0) return 1;
+  if (precision == 64) {
+    auto f = orc_post::god_ray_frame<double>(*cam, *sun);
+    if (consts) { consts[0] = f.sun_ss[0]; consts[1] = f.sun_ss[1]; consts[2] = f.blend; }
+#pragma omp parallel for schedule(dynamic, 4)
+    for (int py = 0; py < H; ++py)
+      for (int px = 0; px < W; ++px)
+        orc_post::god_ray_pixel<double>(f, rgba, mask, W, H, px, py, out + ((size_t)py * W + px) * 4);
+  } else {
+    auto f = orc_post::god_ray_frame<float>(*cam, *sun);
+    if (consts) { consts[0] = f.sun_ss[0]; consts[1] = f.sun_ss[1]; consts[2] = f.blend; }
+#pragma omp parallel for schedule(dynamic, 4)
+    for (int py = 0; py < H; ++py)
+      for (int px = 0; px < W; ++px)
+        orc_post::god_ray_pixel<float>(f, rgba, mask, W, H, px, py, out + ((size_t)py * W + px) * 4);
+  }
+  return 0;
+}
+
+// Tone map of postProcess_ToneMap.frag: rgb (W*H*3) before quantisation
+// and the RGBA8 UNORM store (clamp, x255, round to nearest even).
+int orc_tone_map(uint32_t time_s, int W, int H, const float* rgba, double* rgb, uint8_t* rgba8, int precision) {
+  if (!rgba || W <= 0 || H <= 0) return 1;
+#pragma omp parallel for schedule(static)
+  for (int py = 0; py < H; ++py)
+    for (int px = 0; px < W; ++px) {
+      double o[3];
+      if (precision == 64) {
+        orc_post::tone_map_pixel<double>(rgba, W, H, px, py, time_s, o);
+      } else {
+        float of[3];
+        orc_post::tone_map_pixel<float>(rgba, W, H, px, py, time_s, of);
+        for (int k = 0; k < 3; ++k) o[k] = of[k];
+      }
+      const size_t i = (size_t)py * W + px;
+      for (int k = 0; k < 3; ++k) {
+        if (rgb) rgb[i * 3 + k] = o[k];
+        if (rgba8) {
+          float c = std::fmin(std::fmax((float)o[k], 0.0f), 1.0f);
+          rgba8[i * 4 + k] = (uint8_t)std::nearbyint(c * 255.0f);
+        }
+      }
+      if (rgba8) rgba8[i * 4 + 3] = 255;
+    }
+  return 0;
+}
+
+uint32_t orc_wang_hash(uint32_t u, uint32_t v, uint32_t s) { return orc_post::wang_hash(u, v, s); }
+float orc_uncharted2(float x) { return orc_post::uncharted2<float>(x); }
 
 }  // extern "C"

@@ -106,3 +106,19 @@ def test_missing_library_fails_loudly(monkeypatch, tmp_path):
     monkeypatch.setattr(A, "LIB_PATH", str(tmp_path / "nope.so"))
     with pytest.raises(ImportError):
         A.lib()
+
+
+def test_learned_light_structs_and_null_handling():
+    assert C.sizeof(A.MtcCounters) == 12 * 8
+    assert C.sizeof(A.MtcMlpTrainDesc) == 40 and A.MtcMlpTrainDesc.holdout.offset == 32
+    assert C.sizeof(A.MtcMlpCalibration) == 156 and A.MtcMlpCalibration.mask.offset == 136
+    lib = M.lib()
+    assert lib.mtc_mlp_destroy(None) == A.MTC_OK
+    assert lib.mtc_mlp_init(None, 1) == A.MTC_ERR_INVALID_ARG
+    assert lib.mtc_mlp_predict(None, None, 0, None) == A.MTC_ERR_INVALID_ARG
+    assert lib.mtc_mlp_train(None, None, None) == A.MTC_ERR_INVALID_ARG
+    assert lib.mtc_set_learned_light(None, None, 0) == A.MTC_ERR_INVALID_ARG
+    m = C.c_void_p()
+    assert lib.mtc_mlp_create(0, 0, C.byref(m)) == A.MTC_ERR_INVALID_ARG and not m
+    if M.device_count() == 0:
+        assert lib.mtc_mlp_create(0, 1024, C.byref(m)) == A.MTC_ERR_HIP and not m

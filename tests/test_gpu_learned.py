@@ -1,0 +1,237 @@
+"""Learned light on the GPU, through the C ABI.
+
+  * the network's MFMA inference and its training step's gradient against
+    the numpy oracle (oracle/mlp_oracle.py, quantized to the GPU's BF16);
+  * the training loop on a known smooth function;
+  * the sample generator (the march writing (inputs, -cone optical depth)
+    per in-cloud step) against the CPU oracle's records of the same steps;
+  * what the trained network gains over the cone march in a 1080p frame:
+    cone work removed, frame time, and PSNR vs the FP64 oracle next to the
+    fast kernel's and an untrained network's.
+
+Tolerances: inference within 1e-2 of the output scale (FP32 sums, BF16
+activations whose rounding may flip by one unit); gradient blocks within
+5% relative L2 and cosine >= 0.998 (the GPU keeps dZ and SiLU' in BF16).
+"""
+import numpy as np
+import pytest
+
+import meteoros_amd as M
+from meteoros_amd import _abi as A
+from oracle import mlp_oracle as N
+from oracle import oracle as O
+from tests.common import ground_camera, psnr
+
+pytestmark = pytest.mark.gpu
+
+
+def _params(seed=3):
+    rng = np.random.default_rng(seed)
+    d = {
+        "W1": rng.uniform(-1, 1, (9, 64)) * np.sqrt(3 / 9),
+        "b1": rng.uniform(-0.2, 0.2, 64),
+        "W2": rng.uniform(-1, 1, (64, 64)) * np.sqrt(3 / 64),
+        "b2": rng.uniform(-0.2, 0.2, 64),
+        "W3": rng.uniform(-1, 1, (64, 64)) * np.sqrt(3 / 64),
+        "b3": rng.uniform(-0.2, 0.2, 64),
+        "w4": rng.uniform(-1, 1, 64) * np.sqrt(3 / 64),
+        "b4": 0.3,
+    }
+    return N.pack(d).astype(np.float32)
+
+
+def _inputs(n, seed=5):
+    rng = np.random.default_rng(seed)
+    x = rng.uniform(0, 1, (n, 9)).astype(np.float32)
+    x[:, 5:8] = x[:, 5:8] * 2 - 1
+    return x
+
+
+@pytest.mark.parametrize("n", [1, 17, 64, 1000])
+def test_inference_matches_oracle(n):
+    with M.LightNet(capacity=1024) as net:
+        p = _params(n)
+        net.params = p
+        assert np.array_equal(net.params, p)
+        x = _inputs(n, seed=n)
+        got = net.predict(x)
+    ref = N.forward(p.astype(np.float64), x.astype(np.float64), quantized=True)
+    scale = max(1.0, float(np.abs(ref).max()))
+    err = np.abs(got - ref)
+    print(f"n={n}: max |gpu - oracle| = {err.max():.2e} (output scale {scale:.2f})")
+    assert err.max() <= 1e-2 * scale
+
+
+def test_training_gradient_matches_oracle():
+    with M.LightNet(capacity=1024) as net:
+        p = _params(7)
+        net.params = p
+        x = _inputs(333, seed=9)
+        y = -np.random.default_rng(2).uniform(0, 2.5, 333).astype(np.float32)
+        g, loss = net.gradient(x, y)
+        # a second call gives the same gradient (the accumulator is cleared)
+        g2, _ = net.gradient(x, y)
+    lref, gref = N.loss_and_grad(p.astype(np.float64), x.astype(np.float64), y.astype(np.float64), quantized=True)
+    assert abs(loss - lref) <= 1e-2 * lref, (loss, lref)
+    assert np.allclose(g, g2, rtol=1e-5, atol=1e-7)
+    for block, (o, shape) in N.OFF.items():
+        k = int(np.prod(shape)) if shape else 1
+        a, b = g[o:o + k].astype(np.float64), gref[o:o + k]
+        rel = np.linalg.norm(a - b) / max(np.linalg.norm(b), 1e-12)
+        cos = float(a @ b / max(np.linalg.norm(a) * np.linalg.norm(b), 1e-30))
+        print(f"{block}: rel L2 {rel:.2e}, cos {cos:.6f}")
+        assert rel <= 5e-2, block
+        if k > 1:
+            assert cos >= 0.998, block
+
+
+def test_training_fits_a_smooth_function():
+    rng = np.random.default_rng(4)
+    n, hold = 1 << 16, 8192
+    x = rng.uniform(0, 1, (n, 9)).astype(np.float32)
+    y = -(1.5 * x[:, 4] + 0.5 * np.sin(2 * np.pi * x[:, 0]) * x[:, 3] + 0.3 * x[:, 8] ** 2)
+    with M.LightNet(capacity=n) as net:
+        net.set_samples(x, y.astype(np.float32))
+        assert net.sample_count() == n
+        curve = net.train(steps=800, batch=4096, lr=3e-3, seed=7, holdout=hold)
+        cal = net.calibrate(holdout=hold, max_rms=0.05)
+        pred = net.predict(x[-hold:])
+    first, last = float(curve[:10].mean()), float(curve[-50:].mean())
+    print(f"loss {first:.4f} -> {last:.5f}; held-out rms {cal['rms_all']:.4f} vs mean predictor "
+          f"{cal['rms_mean_predictor']:.4f}")
+    assert np.all(np.isfinite(curve))
+    assert last < 0.05 * first
+    assert cal["rms_all"] < 0.15 * cal["rms_mean_predictor"]
+    rms = float(np.sqrt(np.mean((pred.astype(np.float64) - y[-hold:]) ** 2)))
+    assert abs(rms - cal["rms_all"]) <= 1e-3 * max(1.0, rms)
+    assert sum(cal["count"]) == hold
+
+
+def _scene(W, H, pitch=20.0, yaw=0.0, frame=1, preset=2):
+    params, sun = M.preset_params(preset)
+    cam = ground_camera(W, H, pitch, yaw).ubo()
+    t = M.Time()
+    t.ubo.frame_count_mod16 = frame
+    t.ubo.time[0] = 1.0 / 60.0
+    t.ubo.time[1] = frame / 60.0
+    return cam, t.ubo, sun, params
+
+
+def test_sample_generator_matches_oracle_records():
+    """Every in-cloud step of a 64x36 view (keep fraction 1) against the FP64
+    oracle's records of the same (pixel, step): same steps, same inputs,
+    same cone optical depth up to the fast path's FP16 texels."""
+    W, H = 64, 36
+    dims = (128, 32, 128, 512)
+    cam, tm, sun, params = _scene(W, H, 20.0)
+    with M.CloudStage(W, H, M.VolumeSpec(*dims), kernel=A.MTC_KERNEL_FAST) as st, M.LightNet(1 << 18) as net:
+        st.generate_volumes(1, 2)
+        lf, hf, curl, weather = st.get_volumes()
+        n = net.collect(st, cam, tm, sun, params, seed=1, keep_fraction=1.0)
+        xs, ys, keys = net.read_samples()
+        # a keep fraction of 1/4 keeps about a quarter of them
+        net.clear_samples()
+        nq = net.collect(st, cam, tm, sun, params, seed=1, keep_fraction=0.25)
+    vol = O.Volumes(lf, hf, curl, weather, *dims, 0)
+    xy = np.array([[x, y] for y in range(H) for x in range(W)], dtype=np.int32)
+    pix, step, xo, yo = O.light_samples(vol, cam, tm, sun, params, W, H, xy, precision=64)
+    ko = (xy[pix, 1].astype(np.uint64) << np.uint64(40)) | (xy[pix, 0].astype(np.uint64) << np.uint64(20)) | \
+        step.astype(np.uint64)
+    print(f"gpu {n} samples, oracle {len(ko)}; keep 1/4 -> {nq}")
+    assert n == len(keys) and n > 1000
+    assert len(np.unique(keys)) == n
+    common, ig, io = np.intersect1d(keys, ko, return_indices=True)
+    assert len(common) >= 0.98 * len(ko) and len(common) >= 0.98 * n
+    dx = np.abs(xs[ig].astype(np.float64) - xo[io])
+    dx[:, :3] = np.minimum(dx[:, :3], 1.0 - dx[:, :3])  # positions wrap at the tile edge
+    assert np.quantile(dx.max(axis=1), 0.99) < 2e-3
+    dy = np.abs(ys[ig].astype(np.float64) - yo[io])
+    print(f"target |gpu - oracle|: median {np.median(dy):.2e}, 99% {np.quantile(dy, 0.99):.2e}")
+    assert np.quantile(dy, 0.99) <= 0.02 + 0.02 * np.quantile(np.abs(yo), 0.99)
+    assert 0.2 * n <= nq <= 0.3 * n
+
+
+def _train_on_scenes(net, dims, vol_seed=(1, 2)):
+    """Samples from 12 seeded poses at 480x270 (yaw every 30 deg, pitch
+    8-40 deg), then Adam; returns the calibration dict."""
+    W, H = 480, 270
+    with M.CloudStage(W, H, M.VolumeSpec(*dims), kernel=A.MTC_KERNEL_FAST) as st:
+        st.generate_volumes(*vol_seed)
+        rng = np.random.default_rng(12)
+        for i in range(12):
+            cam, tm, sun, params = _scene(W, H, float(rng.uniform(8, 40)), 30.0 * i, frame=int(rng.integers(16)))
+            net.collect(st, cam, tm, sun, params, seed=100 + i, keep_fraction=0.3)
+    n = net.sample_count()
+    hold = min(65536, n // 8)
+    curve = net.train(steps=3000, batch=8192, lr=2e-3, seed=3, holdout=hold)
+    cal = net.calibrate(holdout=hold, max_rms=0.15, fallback_density=0.8)
+    return n, curve, cal
+
+
+def _frame(kernel, dims, cam, tm, sun, params, net=None, flags=0, frames=1):
+    W, H = 1920, 1080
+    st = M.CloudStage(W, H, M.VolumeSpec(*dims), kernel=kernel, flags=flags, lanes_per_ray=8)
+    st.generate_volumes(1, 2)
+    if net is not None:
+        net.attach(st, audit_period=16)
+    for _ in range(frames):
+        st.frame(cam, cam, tm, sun, params)
+    rgba, _ = st.read_output()
+    out = {"rgba": rgba}
+    if flags & A.MTC_FLAG_COUNTERS:
+        out["counters"] = st.counters()
+    if flags & A.MTC_FLAG_TIMING:
+        t = st.timings()
+        out["march_ms"] = t["march_ms"] / t["frames"]
+    vols = st.get_volumes()
+    st.close()
+    return out, vols
+
+
+def test_learned_light_gain_over_cone_march():
+    dims = (128, 32, 128, 512)
+    with M.LightNet(capacity=1 << 22) as net:
+        n, curve, cal = _train_on_scenes(net, dims)
+        print(f"trained on {n} samples: loss {curve[:20].mean():.4f} -> {curve[-100:].mean():.5f}; held-out rms "
+              f"{cal['rms_all']:.4f} (mean predictor {cal['rms_mean_predictor']:.4f}); bins sent to the cone "
+              f"{bin(cal['mask'])}")
+        assert cal["rms_all"] < 0.5 * cal["rms_mean_predictor"]
+        # an unseen pose of the 1080p workload
+        cam, tm, sun, params = _scene(1920, 1080, 22.0, 15.0, frame=5)
+        fast, vols = _frame(A.MTC_KERNEL_FAST, dims, cam, tm, sun, params, flags=A.MTC_FLAG_COUNTERS)
+        learned, _ = _frame(A.MTC_KERNEL_LEARNED, dims, cam, tm, sun, params, net, flags=A.MTC_FLAG_COUNTERS)
+        tf, _ = _frame(A.MTC_KERNEL_FAST, dims, cam, tm, sun, params, flags=A.MTC_FLAG_TIMING, frames=20)
+        tl, _ = _frame(A.MTC_KERNEL_LEARNED, dims, cam, tm, sun, params, net, flags=A.MTC_FLAG_TIMING, frames=20)
+        untrained_p = N.pack({k: np.zeros(s) if s else 0.0 for k, (_, s) in N.OFF.items()}).astype(np.float32)
+        trained_p = net.params
+        net.params = untrained_p  # predicts 0 optical depth everywhere
+        untrained, _ = _frame(A.MTC_KERNEL_LEARNED, dims, cam, tm, sun, params, net)
+        net.params = trained_p
+    vol = O.Volumes(*vols, *dims, 0)
+    W, H = 1920, 1080
+    xy = O.select_pixels(W, H, 5)[::4]
+    ref, _, _, _ = O.march_pixels(vol, cam, tm, sun, params, W, H, xy, 64)
+    keep = O.outside_sun_disk(cam, tm, params, xy, W, H)
+    pick = lambda r: r[xy[:, 1], xy[:, 0], :3]  # noqa: E731
+    p_fast = O.psnr_vs_fp64(pick(fast["rgba"]), ref[:, :3], keep)
+    p_learned = O.psnr_vs_fp64(pick(learned["rgba"]), ref[:, :3], keep)
+    p_untrained = O.psnr_vs_fp64(pick(untrained["rgba"]), ref[:, :3], keep)
+    cf, cl = fast["counters"], learned["counters"]
+    answered = cl["mlp_queries"] - cl["mlp_fallbacks"]
+    audit_rms = np.sqrt(cl["mlp_audit_sq_err"] * 1e-6 / max(1, cl["mlp_audits"]))
+    print(f"fast:    {tf['march_ms']:.4f} ms march, cone samples {cf['cone_samples']}, PSNR {p_fast:.2f} dB")
+    print(f"learned: {tl['march_ms']:.4f} ms march, cone samples {cl['cone_samples']}, queries {cl['mlp_queries']} "
+          f"(answered {answered}, to the cone {cl['mlp_fallbacks']}), audit rms {audit_rms:.4f} over "
+          f"{cl['mlp_audits']} audits, PSNR {p_learned:.2f} dB")
+    print(f"untrained network: PSNR {p_untrained:.2f} dB; unmasked PSNR fast "
+          f"{psnr(pick(fast['rgba']), ref[:, :3]):.2f} learned {psnr(pick(learned['rgba']), ref[:, :3]):.2f}")
+    # the same rays, the same steps: both kernels see the same in-cloud steps
+    assert cl["cloud_steps"] == cf["cloud_steps"]
+    assert cl["mlp_queries"] == cl["cloud_steps"]
+    # the network answers most steps, so most of the cone work is gone ...
+    assert answered >= 0.5 * cl["mlp_queries"]
+    assert cl["cone_samples"] <= 0.5 * cf["cone_samples"]
+    # ... the cone's audits see the error the held-out set predicted ...
+    assert cl["mlp_audits"] > 0 and audit_rms < 2.0 * cal["rms_all"] + 0.02
+    # ... and training is what makes the answers usable
+    assert p_learned > p_untrained + 3.0
