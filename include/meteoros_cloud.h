@@ -322,6 +322,19 @@ int mtc_reprojection_indices(mtc_ctx* ctx, const mtc_camera* cur, const mtc_came
                              int32_t* idx, size_t n_idx);
 
 /* ------------------------------------------------------------------ */
+/* loaders for the textures Meteoros ships (host only)                 */
+/* ------------------------------------------------------------------ */
+/* One 2D image (TGA or PNG) decoded to RGBA8, top row first, as
+ * stbi_load(path, ..., 4) returns it (Texture2D::createTextureFromFile,
+ * Sky.cpp:44-57).  width/height may be NULL; out may be NULL to query the
+ * size. */
+int mtc_load_image_rgba8(const char* path, int32_t* width, int32_t* height, uint8_t* out, size_t n_bytes);
+/* An n^3 volume from n slices "<dir>/<base>(z+1).tga" into RGBA8 texels,
+ * x fastest, then y, then z (ImageLoadingUtility.cpp:75-139); out holds
+ * n^3 * 4 bytes, ready for mtc_set_volumes. */
+int mtc_load_tga_slices(const char* dir, const char* base, int32_t n, uint8_t* out, size_t n_bytes);
+
+/* ------------------------------------------------------------------ */
 /* post-process passes that consume the stage's output                 */
 /* ------------------------------------------------------------------ */
 enum {

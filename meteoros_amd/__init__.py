@@ -7,7 +7,9 @@ the reference's host objects (Camera, Scene time, Sky/Renderer) over it.
 """
 from ._abi import LIB_PATH, MtcError, lib  # noqa: F401
 from .stage import (Batch, Camera, CloudStage, LightNet, Time, VolumeSpec, device_count, halton,  # noqa: F401
-                    preset_params, reference_params, sun_from_angles)
+                    load_image_rgba8, load_shipped_volumes, load_tga_slices, preset_params, reference_params,
+                    sun_from_angles)
 
 __all__ = ["Batch", "Camera", "CloudStage", "LightNet", "Time", "VolumeSpec", "device_count", "halton", "preset_params",
-           "reference_params", "sun_from_angles", "lib", "LIB_PATH", "MtcError"]
+           "reference_params", "sun_from_angles", "load_image_rgba8", "load_tga_slices", "load_shipped_volumes", "lib",
+           "LIB_PATH", "MtcError"]

@@ -160,6 +160,8 @@ SIGNATURES = [
     ("mtc_reset_timings", C.c_int, [C.c_void_p]),
     ("mtc_reprojection_indices", C.c_int, [C.c_void_p, P(MtcCamera), P(MtcCamera), P(MtcTime), P(MtcCloudParams),
                                            C.c_void_p, C.c_size_t]),
+    ("mtc_load_image_rgba8", C.c_int, [C.c_char_p, P(C.c_int32), P(C.c_int32), C.c_void_p, C.c_size_t]),
+    ("mtc_load_tga_slices", C.c_int, [C.c_char_p, C.c_char_p, C.c_int32, C.c_void_p, C.c_size_t]),
     ("mtc_god_rays", C.c_int, [C.c_void_p, P(MtcCamera), P(MtcSun), C.c_void_p]),
     ("mtc_tone_map", C.c_int, [C.c_void_p, P(MtcTime), C.c_int32, C.c_void_p]),
     ("mtc_read_display", C.c_int, [C.c_void_p, C.c_void_p, C.c_size_t]),
@@ -213,10 +215,18 @@ def lib() -> C.CDLL:
         if not os.path.exists(LIB_PATH):
             raise ImportError(f"{LIB_PATH} is missing: build it with `python -c 'import __graft_entry__ as g; g.build()'`")
         handle = C.CDLL(LIB_PATH)
+        missing = []
         for name, res, args in SIGNATURES:
-            fn = getattr(handle, name)
+            try:
+                fn = getattr(handle, name)
+            except AttributeError:
+                # a library older than this binding: the entry point raises
+                # AttributeError when called; the rest stay usable
+                missing.append(name)
+                continue
             fn.restype = res
             fn.argtypes = args
+        handle.mtc_missing_symbols = missing
         _lib = handle
     return _lib
 

@@ -128,21 +128,28 @@ __device__ __forceinline__ float fast_pow(float x, float y) { return __builtin_a
 __device__ __forceinline__ float fast_exp(float x) { return __builtin_amdgcn_exp2f(x * 1.4426950408889634f); }
 __device__ __forceinline__ float rcp(float x) { return __builtin_amdgcn_rcpf(x); }
 
-// Density of one LF sample (sampleLowFrequency, :499-540).  In weather
-// mode the height gradient's two per-step profiles (sc, st) are passed in.
+// Weather mode: the height-gradient weight of a sample, mix(a, b, type)
+// * 0.5 (getDensityHeightGradientForPoint :475-487 and the x0.5 of the
+// weather path), from the step's two profiles sc, st.
+__device__ __forceinline__ float gradient_weight(float type, float sc, float st) {
+  float a = mixf(st, sc, clampf(type * 2.0f, 0.0f, 1.0f));
+  float b = mixf(sc, st, clampf((type - 0.5f) * 2.0f, 0.0f, 1.0f));
+  return mixf(a, b, type) * 0.5f;
+}
+
+// Density of one LF sample (sampleLowFrequency, :499-540) with coverage
+// `cov`; in weather mode `gw` is the sample's gradient_weight.
+//
+// Exact empty-space test: remapClamped(...) is at most 1, so the weighted
+// base is at most gw, and remapClampedBeforeAndAfter(base, cov, 1, 0, 1)
+// is 0 whenever base <= cov.  So gw <= cov means density 0 without the LF
+// sample; the callers skip the fetch then (bit-identical result).
 template <int COV>
-__device__ __forceinline__ float density(float2 lf, float2 w, float cov_const, float sc, float st) {
+__device__ __forceinline__ float density(float2 lf, float cov, float gw) {
   float fbm = clampf(lf.y, 0.0f, 1.0f);
   // remapClamped(r, fbm - 0.9, 1, 0, 1)
   float base = clampf((lf.x - (fbm - 0.9f)) * rcp(1.9f - fbm), 0.0f, 1.0f);
-  float cov = cov_const;
-  if (COV == MTC_COVERAGE_WEATHER) {
-    float type = w.y;
-    float a = mixf(st, sc, clampf(type * 2.0f, 0.0f, 1.0f));
-    float b = mixf(sc, st, clampf((type - 0.5f) * 2.0f, 0.0f, 1.0f));
-    base = base * (mixf(a, b, type) * 0.5f);
-    cov = w.x;
-  }
+  if (COV == MTC_COVERAGE_WEATHER) base = base * gw;
   // remapClampedBeforeAndAfter(base, cov, 1, 0, 1) * cov
   float v = clampf(base, cov, 1.0f);
   return clampf((v - cov) * rcp(1.0f - cov), 0.0f, 1.0f) * cov;
@@ -267,14 +274,22 @@ __device__ __forceinline__ Prim eval_primary(const FrameConsts& fc, const FastVi
     sc = fmaxf(0.0f, (hc * 4.0f) * ((hc - 0.65f) * (1.0f / -0.35f)));
     st = fmaxf(0.0f, (hc * 10.0f) * ((hc - 0.3f) * (1.0f / -0.1f)));
   }
-  float2 lf = sample_lf<F16>(fv.lf, fv.lf_sh, sk);
-  float2 w = make_float2(0.0f, 0.0f);
-  if (COV == MTC_COVERAGE_WEATHER) w = sample_map<F16>(fv.weather, fv.weather_sh, sp.x * rm.ws, sp.z * rm.ws);
-  const float base = density<COV>(lf, w, fc.coverage_const, sc, st) * fc.base_density_factor;
   o.rel = rel;
   o.h = h;
   o.sc = sc;
   o.st = st;
+  // density, skipping the fetches whose result is known to be 0: outside
+  // the gradient's support (h >= 0.65 or h <= 0, no weather fetch either)
+  // and where the coverage bounds the weighted base (no LF fetch)
+  float cov = fc.coverage_const, gw = 1.0f;
+  if (COV == MTC_COVERAGE_WEATHER) {
+    if (sc == 0.0f && st == 0.0f) return o;
+    const float2 w = sample_map<F16>(fv.weather, fv.weather_sh, sp.x * rm.ws, sp.z * rm.ws);
+    cov = w.x;
+    gw = gradient_weight(w.y, sc, st);
+    if (gw <= cov) return o;
+  }
+  const float base = density<COV>(sample_lf<F16>(fv.lf, fv.lf_sh, sk), cov, gw) * fc.base_density_factor;
   o.base = base;
   if (base > 0.0f) {
     o.cloud = true;
@@ -369,11 +384,14 @@ __device__ __forceinline__ float cone_depth(const FrameConsts& fc, const FastVie
   for (int i = 0; i < fc.light_samples; ++i) {
     const float s = (float)i * cone_scale;
     const f3 slp = mk(pr.rel.x + skern[3 * i] * s, pr.rel.y + skern[3 * i + 1] * s, pr.rel.z + skern[3 * i + 2] * s);
-    float2 clf = sample_lf<false>(fv.lf, fv.lf_sh, slp);
-    float2 cw = make_float2(0.0f, 0.0f);
-    if (COV == MTC_COVERAGE_WEATHER)
-      cw = sample_map<false>(fv.weather, fv.weather_sh, slp.x * fc.weather_scale, slp.z * fc.weather_scale);
-    const float cb = density<COV>(clf, cw, fc.coverage_const, pr.sc, pr.st);
+    float cov = fc.coverage_const, gw = 1.0f;
+    if (COV == MTC_COVERAGE_WEATHER) {
+      const float2 cw =
+          sample_map<false>(fv.weather, fv.weather_sh, slp.x * fc.weather_scale, slp.z * fc.weather_scale);
+      cov = cw.x;
+      gw = gradient_weight(cw.y, pr.sc, pr.st);
+    }
+    const float cb = gw <= cov ? 0.0f : density<COV>(sample_lf<false>(fv.lf, fv.lf_sh, slp), cov, gw);
     dal += cb > 0.0f ? (fc.cone_erosion_scale * cb - pr.m) * pr.inv1m : 0.0f;
   }
   return dal;
@@ -515,11 +533,14 @@ __device__ __forceinline__ void march_tile(const FrameConsts& fc, const FastView
         const float4 a = L.own[r][0], b = L.own[r][1];
         const float s = (float)i * a.w;
         const f3 slp = mk(a.x + skern[3 * i] * s, a.y + skern[3 * i + 1] * s, a.z + skern[3 * i + 2] * s);
-        float2 clf = sample_lf<F16>(fv.lf, fv.lf_sh, slp);
-        float2 cw = make_float2(0.0f, 0.0f);
-        if (COV == MTC_COVERAGE_WEATHER) cw = sample_map<F16>(fv.weather, fv.weather_sh, slp.x * fc.weather_scale,
-                                                               slp.z * fc.weather_scale);
-        const float cb = density<COV>(clf, cw, fc.coverage_const, b.x, b.y);
+        float cov = fc.coverage_const, gw = 1.0f;
+        if (COV == MTC_COVERAGE_WEATHER) {
+          const float2 cw = sample_map<F16>(fv.weather, fv.weather_sh, slp.x * fc.weather_scale,
+                                            slp.z * fc.weather_scale);
+          cov = cw.x;
+          gw = gradient_weight(cw.y, b.x, b.y);
+        }
+        const float cb = gw <= cov ? 0.0f : density<COV>(sample_lf<F16>(fv.lf, fv.lf_sh, slp), cov, gw);
         L.contrib[w] = cb > 0.0f ? (fc.cone_erosion_scale * cb - b.z) * b.w : 0.0f;
         if (COUNT) L.hit[w] = cb > 0.0f ? 1 : 0;
       }

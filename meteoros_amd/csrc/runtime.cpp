@@ -340,9 +340,7 @@ int mtc_frame(mtc_ctx* c, const mtc_camera* cur, const mtc_camera* old, const mt
   }
   if (!full) {
     if (timing) HIP_TRY(hipEventRecord(ev[0], s), "mtc_frame: event");
-    // the optimised kernels pair with the LDS-tiled reprojection; the plain
-    // kernel keeps the one-lane-per-pixel gather
-    HIP_TRY(launch_reproject(fc, ov, c->desc.kernel == MTC_KERNEL_PLAIN ? 0 : 2, s), "reprojection kernel launch");
+    HIP_TRY(launch_reproject(fc, ov, 0, s), "reprojection kernel launch");
   }
   if (timing) HIP_TRY(hipEventRecord(ev[1], s), "mtc_frame: event");
   rc = launch_march(c, fc, ov, full, s);

@@ -131,6 +131,26 @@ int build_frame_consts(const mtc_camera* cur, const mtc_camera* old, const mtc_t
   fc->light_samples = p->light_samples;
   fc->coverage_mode = p->coverage_mode;
   fc->blur_samples = p->motion_blur_samples;
+  {
+    // reprojection.comp castRay (:111-144) and the inner-shell hit's
+    // rO = (rO - c) / R (:147-155): the same FP32 operations the shader
+    // runs per pixel, done once
+    const float* v = cur->view;
+    f3 right = normalize(mk(v[0], v[4], v[8]));
+    f3 up = normalize(mk(v[1], v[5], v[9]));
+    f3 lk = normalize(mk(v[2], v[6], v[10]));
+    f3 look = mk(-lk.x, -lk.y, -lk.z);
+    put(fc->rp_right, right);
+    put(fc->rp_up, up);
+    put(fc->rp_ref, add(eye, look));
+    const float Rin = (float)p->shell_inner_radius;
+    f3 c = mk(eye.x, -(float)p->earth_radius, eye.z);
+    f3 ro = sub(eye, c);
+    ro = mk(ro.x / Rin, ro.y / Rin, ro.z / Rin);
+    put(fc->rp_ro, ro);
+    fc->rp_C = dot(ro, ro) - 1.0f;
+    for (int i = 0; i < p->motion_blur_samples; ++i) fc->rp_frac[i] = (float)i / (float)p->motion_blur_samples;
+  }
   fc->base_density_factor = p->base_density_factor;
   fc->sample_divisor = p->sample_divisor;
   fc->erosion_scale = p->erosion_scale;

@@ -49,6 +49,12 @@ struct FrameConsts {
   float beta_r[3], beta_m[3], beta_sum[3];
   float ydot_clamped;    // clamp((1 - sunDir.y)^5, 0, 1)
   float sun_disk_lo, sun_disk_hi;
+  // reprojection per-frame parts (reprojection.comp:111-144, :147-191, :233)
+  float rp_right[3], rp_up[3];  // normalize(view row 0), normalize(view row 1)
+  float rp_ref[3];              // eye + look, look = -normalize(view row 2)
+  float rp_ro[3];               // (eye - (eye.x, -R_earth, eye.z)) / R_in
+  float rp_C;                   // dot(rp_ro, rp_ro) - 1
+  float rp_frac[64];            // i / motion_blur_samples
 };
 
 // Fills FrameConsts from the boundary structs (runtime_frame.cpp).

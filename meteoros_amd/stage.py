@@ -303,6 +303,36 @@ class Batch:
         return rgba.reshape(h, w, 4), mask.reshape(h, w, 4)
 
 
+def load_image_rgba8(path: str) -> np.ndarray:
+    """A TGA or PNG image as RGBA8 [h, w, 4], top row first (stbi_load(..., 4))."""
+    w, h = C.c_int32(), C.c_int32()
+    check(lib().mtc_load_image_rgba8(path.encode(), C.byref(w), C.byref(h), None, 0), "mtc_load_image_rgba8")
+    out = np.empty((h.value, w.value, 4), dtype=np.uint8)
+    check(lib().mtc_load_image_rgba8(path.encode(), None, None, out.ctypes.data, out.nbytes), "mtc_load_image_rgba8")
+    return out
+
+
+def load_tga_slices(folder: str, base: str, n: int) -> np.ndarray:
+    """An n^3 RGBA8 volume from "<folder>/<base>(z+1).tga" slices, flat, x fastest
+    (ImageLoadingUtility.cpp:75-139)."""
+    out = np.empty(n * n * n * 4, dtype=np.uint8)
+    check(lib().mtc_load_tga_slices(folder.encode(), base.encode(), int(n), out.ctypes.data, out.nbytes),
+          "mtc_load_tga_slices")
+    return out
+
+
+def load_shipped_volumes(texture_dir: str):
+    """The four resources of Sky::CreateCloudResources (Sky.cpp:25-58) from
+    Meteoros' CloudTextures folder: (lf 128^3, hf 32^3, curl 128^2, weather
+    512^2), RGBA8, ready for CloudStage.set_volumes with
+    VolumeSpec(128, 32, 128, 512)."""
+    lf = load_tga_slices(f"{texture_dir}/LowFrequency", "LowFrequency", 128)
+    hf = load_tga_slices(f"{texture_dir}/HighFrequency", "HighFrequency", 32)
+    curl = load_image_rgba8(f"{texture_dir}/curlNoise.png").reshape(-1)
+    weather = load_image_rgba8(f"{texture_dir}/weatherMap.png").reshape(-1)
+    return lf, hf, curl, weather
+
+
 class LightNet:
     """The learned-light network (9 -> 64 -> 64 -> 64 -> 1, SiLU) with its
     device store of training samples.  No reference counterpart: it stands

@@ -28,6 +28,7 @@ def test_library_exports_every_declared_symbol():
     assert not missing, missing
     bound = {s[0] for s in A.SIGNATURES}
     assert set(names) <= bound, set(names) - bound
+    assert lib.mtc_missing_symbols == []
 
 
 def test_std140_layouts():

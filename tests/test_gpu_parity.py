@@ -209,10 +209,9 @@ def test_history_and_error_paths():
 
 
 @pytest.mark.parametrize("yaw,pitch", [(0.5, 0.0), (0.5, 0.4), (12.0, 6.0)])
-def test_tiled_reprojection_of_the_fast_path_is_bit_identical(yaw, pitch):
-    """The fast contexts reproject through the LDS-tiled kernel (staged box
-    for small motion, global fallback for large motion); every pixel the
-    march does not overwrite must equal the plain gather bit for bit."""
+def test_reprojection_of_the_fast_path_is_bit_identical(yaw, pitch):
+    """Small and large camera motion: every pixel the march does not
+    overwrite equals the plain context's and the FP32 oracle's bit for bit."""
     W, H = 640, 360
     params, sun = M.preset_params(2)
     camera = ground_camera(W, H, 20.0)
