@@ -517,7 +517,8 @@ __device__ __forceinline__ void march_tile(const FrameConsts& fc, const FastView
     }
     const bool run_cone = need_cone || audit;
     const unsigned long long bal = __ballot(run_cone);
-    if (__ballot(pr.cloud)) {
+    const bool any_cloud = __ballot(pr.cloud) != 0ull;
+    if (any_cloud) {
       const int ncl = __popcll(bal);
       const int rank = __popcll(bal & ((1ull << lane) - 1ull));
       if (run_cone) {
@@ -580,7 +581,15 @@ __device__ __forceinline__ void march_tile(const FrameConsts& fc, const FastView
     }
 
     // accumulation / early exit in step order (:674-684)
-    if (!gdone) {
+    if (!gdone && !any_cloud) {
+      // no in-cloud step in the wave: nothing accumulates, so only the
+      // step count and the validity of the group's last step change (valid
+      // steps form a prefix of the chunk: t_k grows with k)
+      const unsigned long long vb = __ballot(pr.valid);
+      const unsigned int gm = (unsigned int)((vb >> gbase) & ((1ull << G) - 1ull));
+      if (COUNT && j == 0) n_steps += (unsigned)__popc(gm);
+      gdone = ((gm >> (G - 1)) & 1u) == 0u;
+    } else if (!gdone) {
       bool last_valid = false, exited = false;
 #pragma unroll
       for (int i = 0; i < G; ++i) {
