@@ -240,7 +240,7 @@ typedef struct mtc_counters {
   uint64_t rays_sky;         /* 0 <= dir.y < fade start early-outs    */
   uint64_t primary_steps;    /* executed primary iterations (ray-steps) */
   uint64_t cloud_steps;      /* primary iterations with base density > 0 */
-  uint64_t cone_samples;     /* cone LF samples                       */
+  uint64_t cone_samples;     /* cone samples (learned frames: 1 per answered step) */
   uint64_t cone_erosions;    /* cone samples with density > 0          */
   uint64_t early_exits;      /* accumDensity >= 1 breaks              */
   uint64_t mlp_queries;      /* learned-light queries                 */

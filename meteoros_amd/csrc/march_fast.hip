@@ -350,51 +350,49 @@ __device__ __forceinline__ void setup_march(const FrameConsts& fc, const Ray& ra
     rm.cap = 4 * max(fc.steps_min, fc.steps_max) + 64;
 }
 
+// One density term of the cone light march (:348-369) for sample i of an
+// in-cloud step: density at rel + kernel_i * i * cone_scale (unskewed,
+// undivided), eroded with the step's own high-frequency modifier.
+template <int COV>
+__device__ __forceinline__ float cone_term(const FrameConsts& fc, const FastViews& fv, const float* skern,
+                                          const Prim& pr, float cone_scale, int i) {
+  const float s = (float)i * cone_scale;
+  const f3 slp = mk(pr.rel.x + skern[3 * i] * s, pr.rel.y + skern[3 * i + 1] * s, pr.rel.z + skern[3 * i + 2] * s);
+  float cov = fc.coverage_const, gw = 1.0f;
+  if (COV == MTC_COVERAGE_WEATHER) {
+    const float2 cw = sample_map<false>(fv.weather, fv.weather_sh, slp.x * fc.weather_scale, slp.z * fc.weather_scale);
+    cov = cw.x;
+    gw = gradient_weight(cw.y, pr.sc, pr.st);
+  }
+  const float cb = gw <= cov ? 0.0f : density<COV>(sample_lf<false>(fv.lf, fv.lf_sh, slp), cov, gw);
+  return cb > 0.0f ? (fc.cone_erosion_scale * cb - pr.m) * pr.inv1m : 0.0f;
+}
+
 // The network's nine inputs for one in-cloud step (learned light):
 //   0-2  position in the shell, as the cone sees it: the cone samples the
 //        LF volume at rel + offsets (period 1) and the weather map at
 //        (rel.x, rel.z) * weather_scale, so x and z are taken modulo the
 //        weather tile, y modulo the LF tile
 //   3    height fraction
-//   4    local density (sampleLowFrequency, before base_density_factor)
+//   4    local density as the light march sees it: the cone's first term
+//        (its sample at the step itself), computed exactly
 //   5-7  sun direction
 //   8    cos of the view ray's zenith angle: with the camera in the
 //        atmosphere it fixes the shell span, hence the primary step and the
 //        cone's length (step / thickness)
-__device__ __forceinline__ void mlp_features(const FrameConsts& fc, const Prim& pr, const RayMarch& rm, float f[9]) {
+// The network answers the rest of the cone: -(terms 1 .. n-1).
+__device__ __forceinline__ void mlp_features(const FrameConsts& fc, const Prim& pr, const RayMarch& rm, float c0,
+                                             float f[9]) {
   const float ax = pr.rel.x * rm.ws, az = pr.rel.z * rm.ws;
   f[0] = ax - floorf(ax);
   f[1] = pr.rel.y - floorf(pr.rel.y);
   f[2] = az - floorf(az);
   f[3] = pr.h;
-  f[4] = pr.base / fc.base_density_factor;
+  f[4] = c0;
   f[5] = fc.light_dir[0];
   f[6] = fc.light_dir[1];
   f[7] = fc.light_dir[2];
   f[8] = rm.ray.d.y;
-}
-
-// Cone light march of one in-cloud step, one lane (GetLightEnergy's
-// optical depth, :338-373): the same samples and summation order as the
-// compacted march.
-template <int COV>
-__device__ __forceinline__ float cone_depth(const FrameConsts& fc, const FastViews& fv, const float* skern,
-                                           const Prim& pr, float cone_scale) {
-  float dal = 0.0f;
-  for (int i = 0; i < fc.light_samples; ++i) {
-    const float s = (float)i * cone_scale;
-    const f3 slp = mk(pr.rel.x + skern[3 * i] * s, pr.rel.y + skern[3 * i + 1] * s, pr.rel.z + skern[3 * i + 2] * s);
-    float cov = fc.coverage_const, gw = 1.0f;
-    if (COV == MTC_COVERAGE_WEATHER) {
-      const float2 cw =
-          sample_map<false>(fv.weather, fv.weather_sh, slp.x * fc.weather_scale, slp.z * fc.weather_scale);
-      cov = cw.x;
-      gw = gradient_weight(cw.y, pr.sc, pr.st);
-    }
-    const float cb = gw <= cov ? 0.0f : density<COV>(sample_lf<false>(fv.lf, fv.lf_sh, slp), cov, gw);
-    dal += cb > 0.0f ? (fc.cone_erosion_scale * cb - pr.m) * pr.inv1m : 0.0f;
-  }
-  return dal;
 }
 
 __device__ __forceinline__ uint32_t step_hash(uint32_t a, uint32_t b, uint32_t c, uint32_t seed) {
@@ -473,7 +471,8 @@ __device__ __forceinline__ void march_tile(const FrameConsts& fc, const FastView
     float dal_mlp = 0.0f;
     bool need_cone = pr.cloud;
     bool audit = false;
-    if (LEARN) {
+    float c0 = 0.0f;
+    if (LEARN && nls > 0) {
       // every in-cloud step queries the network (one row per step, ranked
       // by ballot); steps it should not be trusted with go to the cone
       const unsigned long long qbal = __ballot(pr.cloud);
@@ -482,8 +481,9 @@ __device__ __forceinline__ void march_tile(const FrameConsts& fc, const FastView
         const int nq = __popcll(qbal);
         const int qrank = __popcll(qbal & ((1ull << lane) - 1ull));
         if (pr.cloud) {
+          c0 = cone_term<COV>(fc, fv, skern, pr, rm.cone_scale, 0);
           float f[9];
-          mlp_features(fc, pr, rm, f);
+          mlp_features(fc, pr, rm, c0, f);
           uint32_t w[8];
 #pragma unroll
           for (int k = 0; k < 8; ++k) {
@@ -504,7 +504,7 @@ __device__ __forceinline__ void march_tile(const FrameConsts& fc, const FastView
           bin = bin < 0 ? 0 : (bin > 15 ? 15 : bin);
           const bool fb = pr.base / fc.base_density_factor > fv.fallback_density || ((fv.calib_mask >> bin) & 1u);
           if (!fb) {
-            dal_mlp = -ym;
+            dal_mlp = c0 - ym;
             need_cone = false;
             audit = fv.audit && (step_hash((uint32_t)px, (uint32_t)py, (uint32_t)(k0 + j), 0u) & (uint32_t)(fv.audit - 1)) == 0u;
           }
@@ -608,6 +608,8 @@ __device__ __forceinline__ void march_tile(const FrameConsts& fc, const FastView
               if (ch >= 0) {
                 n_cone += (unsigned)nls;
                 n_cone_er += (unsigned)ch;
+              } else if (LEARN) {
+                n_cone += 1u;  // the network's input term, the cone's first sample
               }
               if (LEARN) {
                 n_mlp++;
@@ -696,8 +698,8 @@ __global__ __launch_bounds__(256) void march_learned_kernel(FrameConsts fc, Fast
 // Training samples for the learned light: every pixel of the view is
 // marched (one lane per ray, the same steps and early exit as the frame
 // kernels); each in-cloud step is kept with probability keep / 2^32 and
-// written as (nine inputs, -cone optical depth, key = py << 40 | px << 20 |
-// step) at an atomically claimed slot.  Slots past `cap` are counted but
+// written as (nine inputs, -(cone terms 1 .. n-1), key = py << 40 | px << 20
+// | step) at an atomically claimed slot.  Slots past `cap` are counted but
 // not written.
 template <int COV>
 __global__ __launch_bounds__(256) void light_samples_kernel(FrameConsts fc, FastViews fv, uint32_t seed, uint32_t keep,
@@ -725,13 +727,15 @@ __global__ __launch_bounds__(256) void light_samples_kernel(FrameConsts fc, Fast
     if (pr.cloud) {
       accum += pr.da;
       if (step_hash((uint32_t)px, (uint32_t)py, (uint32_t)k, seed) < keep) {
+        const float c0 = fc.light_samples > 0 ? cone_term<COV>(fc, fv, skern, pr, rm.cone_scale, 0) : 0.0f;
+        float rest = 0.0f;
+        for (int i = 1; i < fc.light_samples; ++i) rest += cone_term<COV>(fc, fv, skern, pr, rm.cone_scale, i);
         float f[9];
-        mlp_features(fc, pr, rm, f);
-        const float dal = cone_depth<COV>(fc, fv, skern, pr, rm.cone_scale);
+        mlp_features(fc, pr, rm, c0, f);
         const unsigned long long slot = atomicAdd(count, 1ull);
         if (slot < cap) {
           for (int i = 0; i < 9; ++i) xs[slot * 9 + i] = f[i];
-          ys[slot] = -dal;
+          ys[slot] = -rest;
           keys[slot] = ((unsigned long long)py << 40) | ((unsigned long long)px << 20) | (unsigned long long)k;
         }
       }

@@ -42,11 +42,8 @@ constexpr int OW1 = 0, OB1 = 576, OW2 = 640, OB2 = 4736, OW3 = 4800, OB3 = 8896,
 // per output tile (nt), layers 2 and 3 one per (nt, k-chunk)
 constexpr int kFragL1 = 0, kFragL2 = 4 * 64, kFragL3 = 12 * 64, kFrags = 20 * 64;
 
-__device__ __forceinline__ uint16_t f2bf(float f) {
-  uint32_t u = __builtin_bit_cast(uint32_t, f);
-  u += 0x7fffu + ((u >> 16) & 1u);  // round to nearest even (finite inputs)
-  return (uint16_t)(u >> 16);
-}
+// FP32 -> BF16, round to nearest even (one v_cvt_pk_bf16_f32 on gfx950)
+__device__ __forceinline__ uint16_t f2bf(float f) { return __builtin_bit_cast(uint16_t, (__bf16)f); }
 __device__ __forceinline__ float bf2f(uint16_t h) { return __builtin_bit_cast(float, (uint32_t)h << 16); }
 __device__ __forceinline__ __bf16 bfv(uint16_t h) { return __builtin_bit_cast(__bf16, h); }
 
@@ -105,8 +102,8 @@ __device__ __forceinline__ bf16x8 activate(floatx4 z0, floatx4 z1) {
   bf16x8 h;
 #pragma unroll
   for (int r = 0; r < 4; ++r) {
-    h[r] = bfv(f2bf(silu(z0[r])));
-    h[4 + r] = bfv(f2bf(silu(z1[r])));
+    h[r] = (__bf16)silu(z0[r]);
+    h[4 + r] = (__bf16)silu(z1[r]);
   }
   return h;
 }

@@ -447,12 +447,15 @@ static inline T erode(const Frame<T>& f, const Volumes& vol, T base, v3<T> p, T 
 }
 
 // One in-cloud step as the learned light sees it: its step index, the nine
-// network inputs and the cone's log transmittance -dal (the training
-// target).  No GLSL counterpart; recorded by ray_march when asked.
+// network inputs (the fifth is the cone's first density term) and the
+// training target, the log transmittance of the rest of the cone,
+// -(terms 1 .. n-1).  No GLSL counterpart; recorded by ray_march when asked.
 struct LightRecord {
   int k;
   double x[9];
   double y;
+  double c[6];     // the cone's density terms (0 where the sample is empty)
+  double w[2];     // weather (coverage, type) at the cone's first sample
 };
 
 // rayMarch, :565-688
@@ -509,6 +512,7 @@ static v3<T> ray_march(const Frame<T>& f, const Volumes& vol, const Ray<T>& ray,
       T hfd = erode(f, vol, base * f.erosion_scale, sk, h);
       accum += hfd * f.accum_scale;
       T dal = T(0);
+      T terms[6] = {T(0), T(0), T(0), T(0), T(0), T(0)};
       for (int i = 0; i < f.light_samples; ++i) {
         cnt.cone_samples++;
         v3<T> lp = pos + ((stepSize * kern[i]) * (T)i);
@@ -516,13 +520,15 @@ static v3<T> ray_march(const Frame<T>& f, const Volumes& vol, const Ray<T>& ray,
         T cb = sample_low(f, vol, slp, slp, h);
         if (cb > T(0)) {
           cnt.cone_erosions++;
-          dal += erode(f, vol, f.cone_erosion_scale * cb, sk, h);
+          T e = erode(f, vol, f.cone_erosion_scale * cb, sk, h);
+          if (i < 6) terms[i] = e;
+          dal += e;
         }
       }
       if (rec) {
         // inputs: position in the shell (x, z modulo the weather tile, y
-        // modulo the LF tile), height, local density, sun direction, cos of
-        // the ray's zenith angle
+        // modulo the LF tile), height, the cone's first term, sun direction,
+        // cos of the ray's zenith angle
         v3<T> rel = rel_pos(f, pos, c);
         LightRecord r;
         r.k = iter;
@@ -530,12 +536,19 @@ static v3<T> ray_march(const Frame<T>& f, const Volumes& vol, const Ray<T>& ray,
         r.x[1] = (double)fractT(rel.y);
         r.x[2] = (double)fractT(rel.z * f.weather_scale);
         r.x[3] = (double)h;
-        r.x[4] = (double)(base / f.base_density_factor);
+        r.x[4] = (double)terms[0];  // local density as the light march sees it: its first term
         r.x[5] = (double)lightDir.x;
         r.x[6] = (double)lightDir.y;
         r.x[7] = (double)lightDir.z;
         r.x[8] = (double)ray.d.y;
-        r.y = -(double)dal;
+        // target: the rest of the cone, -(terms 1 .. n-1)
+        T rest = T(0);
+        for (int i = 1; i < f.light_samples && i < 6; ++i) rest = rest + terms[i];
+        r.y = -(double)rest;
+        for (int i = 0; i < 6; ++i) r.c[i] = (double)terms[i];
+        C4<T> w0 = sample2d<T>(vol.weather, vol.weather_dim, rel.x * f.weather_scale, rel.z * f.weather_scale);
+        r.w[0] = (double)w0.r;
+        r.w[1] = (double)w0.g;
         rec->push_back(r);
       }
       T E = light_energy(h, dal, base, HG_light, cos_angle, stepSize, f.brightness);

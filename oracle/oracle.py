@@ -41,7 +41,7 @@ def lib() -> C.CDLL:
         h.orc_light_samples.restype = C.c_int64
         h.orc_light_samples.argtypes = [vp, vp, vp, vp, P(A.MtcVolumeDesc), P(A.MtcCamera), P(A.MtcTime),
                                         P(A.MtcSun), P(A.MtcCloudParams), C.c_int, C.c_int, vp, C.c_int64, C.c_int,
-                                        C.c_int64, vp, vp, vp, vp]
+                                        C.c_int64, vp, vp, vp, vp, vp]
         h.orc_god_rays.restype = C.c_int
         h.orc_god_rays.argtypes = [P(A.MtcCamera), P(A.MtcSun), C.c_int, C.c_int, vp, vp, vp, C.c_int, vp]
         h.orc_tone_map.restype = C.c_int
@@ -149,10 +149,12 @@ def march_pixels(vol: Volumes, cam, time, sun, params, W, H, xy: np.ndarray, pre
     return rgba, mask, steps, dict(zip(COUNTER_NAMES, (int(c) for c in cnt)))
 
 
-def light_samples(vol: Volumes, cam, time, sun, params, W, H, xy: np.ndarray, precision=64):
+def light_samples(vol: Volumes, cam, time, sun, params, W, H, xy: np.ndarray, precision=64, extra=False):
     """Learned-light training samples of the listed pixels (every in-cloud
     step).  Returns (pixel index into xy [m], step index [m], inputs [m, 9],
-    target [m]) with target = -(cone optical depth)."""
+    target [m]) with target = -(cone optical depth); with extra=True also
+    [m, 8]: the six cone terms and the weather (coverage, type) at the
+    cone's first sample."""
     xy = np.ascontiguousarray(xy, dtype=np.int32).reshape(-1, 2)
     n = xy.shape[0]
     cap = max(1, n * 64)
@@ -161,13 +163,15 @@ def light_samples(vol: Volumes, cam, time, sun, params, W, H, xy: np.ndarray, pr
         step = np.empty(cap, dtype=np.int32)
         x = np.empty((cap, 9), dtype=np.float64)
         y = np.empty(cap, dtype=np.float64)
+        ex = np.empty((cap, 8), dtype=np.float64)
         m = lib().orc_light_samples(*vol.ptrs(), C.byref(vol.desc), C.byref(cam), C.byref(time), C.byref(sun),
                                     C.byref(params), W, H, xy.ctypes.data, n, precision, cap, pix.ctypes.data,
-                                    step.ctypes.data, x.ctypes.data, y.ctypes.data)
+                                    step.ctypes.data, x.ctypes.data, y.ctypes.data, ex.ctypes.data)
         if m < 0:
             raise ValueError("orc_light_samples rejected its arguments")
         if m <= cap:
-            return pix[:m], step[:m], x[:m], y[:m]
+            out = (pix[:m], step[:m], x[:m], y[:m])
+            return out + (ex[:m],) if extra else out
         cap = int(m)
 
 

@@ -87,12 +87,14 @@ int orc_march_pixels(const void* lf, const void* hf, const uint8_t* curl, const 
 // The learned light's training samples of the listed pixels, marched in
 // full (every pixel's own ray, pixelID = frame_count_mod16 as in the march):
 // for every in-cloud step its pixel, step index, nine network inputs and the
-// cone's log transmittance.  Writes up to `cap` records, returns how many
-// there are.  Single-threaded (small pixel lists).
+// cone's log transmittance; `extra` (may be NULL, 8 doubles per record)
+// the six cone terms and the weather (coverage, type) at the cone's first
+// sample.  Writes up to `cap` records, returns how many there are.
+// Single-threaded (small pixel lists).
 int64_t orc_light_samples(const void* lf, const void* hf, const uint8_t* curl, const uint8_t* weather,
                           const mtc_volume_desc* vd, const mtc_camera* cam, const mtc_time* tm, const mtc_sun* sun,
                           const mtc_cloud_params* p, int W, int H, const int32_t* xy, int64_t n, int precision,
-                          int64_t cap, int32_t* pix, int32_t* step, double* x, double* y) {
+                          int64_t cap, int32_t* pix, int32_t* step, double* x, double* y, double* extra) {
   if (!lf || !hf || !curl || !weather || !vd || !cam || !tm || !sun || !p || !xy || W <= 0 || H <= 0) return -1;
   Volumes vol = make_volumes(lf, hf, curl, weather, vd);
   const int pid = tm->frame_count_mod16;
@@ -116,6 +118,11 @@ int64_t orc_light_samples(const void* lf, const void* hf, const uint8_t* curl, c
         step[total] = r.k;
         for (int j = 0; j < 9; ++j) x[total * 9 + j] = r.x[j];
         y[total] = r.y;
+        if (extra) {
+          for (int j = 0; j < 6; ++j) extra[total * 8 + j] = r.c[j];
+          extra[total * 8 + 6] = r.w[0];
+          extra[total * 8 + 7] = r.w[1];
+        }
       }
       ++total;
     }

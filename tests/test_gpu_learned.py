@@ -3,11 +3,11 @@
   * the network's MFMA inference and its training step's gradient against
     the numpy oracle (oracle/mlp_oracle.py, quantized to the GPU's BF16);
   * the training loop on a known smooth function;
-  * the sample generator (the march writing (inputs, -cone optical depth)
+  * the sample generator (the march writing (inputs, -(cone terms 1..5))
     per in-cloud step) against the CPU oracle's records of the same steps;
   * what the trained network gains over the cone march in a 1080p frame:
-    cone work removed, frame time, and PSNR vs the FP64 oracle next to the
-    fast kernel's and an untrained network's.
+    cone work removed, march time, and PSNR vs the FP64 oracle next to the
+    fast kernel's and the zeroed network's (the cone's first term alone).
 
 Tolerances: inference within 1e-2 of the output scale (FP32 sums, BF16
 activations whose rounding may flip by one unit); gradient blocks within
@@ -144,27 +144,36 @@ def test_sample_generator_matches_oracle_records():
     assert len(common) >= 0.98 * len(ko) and len(common) >= 0.98 * n
     dx = np.abs(xs[ig].astype(np.float64) - xo[io])
     dx[:, :3] = np.minimum(dx[:, :3], 1.0 - dx[:, :3])  # positions wrap at the tile edge
-    assert np.quantile(dx.max(axis=1), 0.99) < 2e-3
+    geo = np.delete(dx, 4, axis=1)  # position, height, sun, ray: geometry only
+    assert np.quantile(geo.max(axis=1), 0.99) < 2e-3
+    # input 4 is the cone's first density term (FP16 texels on the GPU)
+    assert np.quantile(dx[:, 4], 0.99) <= 0.01 + 0.01 * np.quantile(np.abs(xo[io, 4]), 0.99)
     dy = np.abs(ys[ig].astype(np.float64) - yo[io])
     print(f"target |gpu - oracle|: median {np.median(dy):.2e}, 99% {np.quantile(dy, 0.99):.2e}")
     assert np.quantile(dy, 0.99) <= 0.02 + 0.02 * np.quantile(np.abs(yo), 0.99)
     assert 0.2 * n <= nq <= 0.3 * n
 
 
-def _train_on_scenes(net, dims, vol_seed=(1, 2)):
+def _train_on_scenes(net, dims, vol_seed=(1, 2), steps=4000):
     """Samples from 12 seeded poses at 480x270 (yaw every 30 deg, pitch
-    8-40 deg), then Adam; returns the calibration dict."""
+    8-40 deg), then Adam; returns (samples, loss curve, calibration)."""
+    import time
+
     W, H = 480, 270
+    t0 = time.perf_counter()
     with M.CloudStage(W, H, M.VolumeSpec(*dims), kernel=A.MTC_KERNEL_FAST) as st:
         st.generate_volumes(*vol_seed)
         rng = np.random.default_rng(12)
         for i in range(12):
             cam, tm, sun, params = _scene(W, H, float(rng.uniform(8, 40)), 30.0 * i, frame=int(rng.integers(16)))
             net.collect(st, cam, tm, sun, params, seed=100 + i, keep_fraction=0.3)
+    t1 = time.perf_counter()
     n = net.sample_count()
     hold = min(65536, n // 8)
-    curve = net.train(steps=3000, batch=8192, lr=2e-3, seed=3, holdout=hold)
-    cal = net.calibrate(holdout=hold, max_rms=0.15, fallback_density=0.8)
+    curve = net.train(steps=steps, batch=8192, lr=2e-3, seed=3, holdout=hold)
+    t2 = time.perf_counter()
+    cal = net.calibrate(holdout=hold, max_rms=0.25, fallback_density=0.8)
+    print(f"collected {n} samples in {t1 - t0:.2f} s, trained {steps} x 8192 in {t2 - t1:.2f} s")
     return n, curve, cal
 
 
@@ -189,23 +198,27 @@ def _frame(kernel, dims, cam, tm, sun, params, net=None, flags=0, frames=1):
 
 
 def test_learned_light_gain_over_cone_march():
+    """What the trained network gains: the same 1080p frame marched with the
+    full cone (fast kernel), with the learned light, and with the learned
+    light's network zeroed (each answered step then keeps only the cone's
+    first term, the network's own input).  Reports cone work, march time and
+    PSNR vs the FP64 oracle for each."""
     dims = (128, 32, 128, 512)
     with M.LightNet(capacity=1 << 22) as net:
         n, curve, cal = _train_on_scenes(net, dims)
         print(f"trained on {n} samples: loss {curve[:20].mean():.4f} -> {curve[-100:].mean():.5f}; held-out rms "
               f"{cal['rms_all']:.4f} (mean predictor {cal['rms_mean_predictor']:.4f}); bins sent to the cone "
               f"{bin(cal['mask'])}")
-        assert cal["rms_all"] < 0.5 * cal["rms_mean_predictor"]
+        assert cal["rms_all"] < 0.6 * cal["rms_mean_predictor"]
         # an unseen pose of the 1080p workload
         cam, tm, sun, params = _scene(1920, 1080, 22.0, 15.0, frame=5)
         fast, vols = _frame(A.MTC_KERNEL_FAST, dims, cam, tm, sun, params, flags=A.MTC_FLAG_COUNTERS)
         learned, _ = _frame(A.MTC_KERNEL_LEARNED, dims, cam, tm, sun, params, net, flags=A.MTC_FLAG_COUNTERS)
         tf, _ = _frame(A.MTC_KERNEL_FAST, dims, cam, tm, sun, params, flags=A.MTC_FLAG_TIMING, frames=20)
         tl, _ = _frame(A.MTC_KERNEL_LEARNED, dims, cam, tm, sun, params, net, flags=A.MTC_FLAG_TIMING, frames=20)
-        untrained_p = N.pack({k: np.zeros(s) if s else 0.0 for k, (_, s) in N.OFF.items()}).astype(np.float32)
         trained_p = net.params
-        net.params = untrained_p  # predicts 0 optical depth everywhere
-        untrained, _ = _frame(A.MTC_KERNEL_LEARNED, dims, cam, tm, sun, params, net)
+        net.params = np.zeros_like(trained_p)  # network output 0: the cone's first term alone
+        first_only, _ = _frame(A.MTC_KERNEL_LEARNED, dims, cam, tm, sun, params, net)
         net.params = trained_p
     vol = O.Volumes(*vols, *dims, 0)
     W, H = 1920, 1080
@@ -215,7 +228,7 @@ def test_learned_light_gain_over_cone_march():
     pick = lambda r: r[xy[:, 1], xy[:, 0], :3]  # noqa: E731
     p_fast = O.psnr_vs_fp64(pick(fast["rgba"]), ref[:, :3], keep)
     p_learned = O.psnr_vs_fp64(pick(learned["rgba"]), ref[:, :3], keep)
-    p_untrained = O.psnr_vs_fp64(pick(untrained["rgba"]), ref[:, :3], keep)
+    p_first = O.psnr_vs_fp64(pick(first_only["rgba"]), ref[:, :3], keep)
     cf, cl = fast["counters"], learned["counters"]
     answered = cl["mlp_queries"] - cl["mlp_fallbacks"]
     audit_rms = np.sqrt(cl["mlp_audit_sq_err"] * 1e-6 / max(1, cl["mlp_audits"]))
@@ -223,15 +236,14 @@ def test_learned_light_gain_over_cone_march():
     print(f"learned: {tl['march_ms']:.4f} ms march, cone samples {cl['cone_samples']}, queries {cl['mlp_queries']} "
           f"(answered {answered}, to the cone {cl['mlp_fallbacks']}), audit rms {audit_rms:.4f} over "
           f"{cl['mlp_audits']} audits, PSNR {p_learned:.2f} dB")
-    print(f"untrained network: PSNR {p_untrained:.2f} dB; unmasked PSNR fast "
-          f"{psnr(pick(fast['rgba']), ref[:, :3]):.2f} learned {psnr(pick(learned['rgba']), ref[:, :3]):.2f}")
+    print(f"first cone term only (network zeroed): PSNR {p_first:.2f} dB")
     # the same rays, the same steps: both kernels see the same in-cloud steps
     assert cl["cloud_steps"] == cf["cloud_steps"]
     assert cl["mlp_queries"] == cl["cloud_steps"]
     # the network answers most steps, so most of the cone work is gone ...
     assert answered >= 0.5 * cl["mlp_queries"]
-    assert cl["cone_samples"] <= 0.5 * cf["cone_samples"]
+    assert cl["cone_samples"] <= 0.6 * cf["cone_samples"]
     # ... the cone's audits see the error the held-out set predicted ...
     assert cl["mlp_audits"] > 0 and audit_rms < 2.0 * cal["rms_all"] + 0.02
-    # ... and training is what makes the answers usable
-    assert p_learned > p_untrained + 3.0
+    # ... and the trained network is what makes the answers usable
+    assert p_learned > p_first + 3.0
