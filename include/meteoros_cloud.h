@@ -353,6 +353,18 @@ int mtc_god_rays(mtc_ctx* ctx, const mtc_camera* cam, const mtc_sun* sun, void* 
 int mtc_tone_map(mtc_ctx* ctx, const mtc_time* time, int32_t format, void* stream);
 /* Copy the display image (W*H*4 bytes for RGBA8, W*H*16 for RGBA32F). */
 int mtc_read_display(mtc_ctx* ctx, void* out, size_t n_bytes);
+/* TXAA (postProcess_TXAA.frag:171-270; Renderer.cpp:840-846) of the RGBA8
+ * display image against the previous TXAA output: reprojection through the
+ * shader's own inner shell (R + 7.5 km), 3x3 / cross neighbourhood clip,
+ * luminance-weighted feedback in [0, 0.5].  The result (RGBA8) becomes the
+ * next call's history; the first history is black.  Needs an RGBA8
+ * mtc_tone_map first.  The reference blends in place, reading neighbours
+ * other fragments may have overwritten; here the current image is read-only. */
+int mtc_txaa(mtc_ctx* ctx, const mtc_camera* cur, const mtc_camera* old, const mtc_time* time, void* stream);
+/* Copy the latest TXAA output (W*H*4 bytes). */
+int mtc_read_txaa(mtc_ctx* ctx, uint8_t* rgba8, size_t n_bytes);
+/* Replace the TXAA history (W*H*4 bytes). */
+int mtc_set_txaa_history(mtc_ctx* ctx, const uint8_t* rgba8, size_t n_bytes);
 
 /* ------------------------------------------------------------------ */
 /* batched views: many views of uneven size in one persistent launch  */

@@ -165,6 +165,9 @@ SIGNATURES = [
     ("mtc_god_rays", C.c_int, [C.c_void_p, P(MtcCamera), P(MtcSun), C.c_void_p]),
     ("mtc_tone_map", C.c_int, [C.c_void_p, P(MtcTime), C.c_int32, C.c_void_p]),
     ("mtc_read_display", C.c_int, [C.c_void_p, C.c_void_p, C.c_size_t]),
+    ("mtc_txaa", C.c_int, [C.c_void_p, P(MtcCamera), P(MtcCamera), P(MtcTime), C.c_void_p]),
+    ("mtc_read_txaa", C.c_int, [C.c_void_p, C.c_void_p, C.c_size_t]),
+    ("mtc_set_txaa_history", C.c_int, [C.c_void_p, C.c_void_p, C.c_size_t]),
     ("mtc_batch_create", C.c_int, [C.c_void_p, C.c_int32, P(C.c_int32), P(C.c_int32), P(C.c_void_p)]),
     ("mtc_batch_destroy", C.c_int, [C.c_void_p]),
     ("mtc_batch_set_tuning", C.c_int, [C.c_void_p, P(MtcBatchTuning)]),

@@ -18,6 +18,13 @@
 // the 100 samples is one 16-byte load instead of four.  Sampler: linear,
 // CLAMP_TO_BORDER with opaque black (Texture2D.cpp:75, Image.cpp:305-345).
 //
+// TXAA.  Re-projects each pixel into the previous TXAA output through the
+// inner atmosphere shell of the TXAA shader (R + 7.5 km), clips the
+// history to the current frame's 3x3 / cross neighbourhood box and blends
+// with a luminance-weighted feedback in [0, 0.5].  The reference reads and
+// writes the current image in place (neighbours race); here the current
+// frame is read-only and the result goes to the other history buffer.
+//
 // Tone map.  Bilinear read of the HDR result at the pixel centre (same
 // sampler), Uncharted-2 curve with exposure 2.5 and white point 100, gamma
 // 1/2.2, plus 0.01 x WangHashNoise(x, y, uint(total time)); stored as
@@ -143,6 +150,125 @@ __global__ __launch_bounds__(256) void tone_map_kernel(const float4* __restrict_
   }
 }
 
+// ---------------------------------------------------------------------------
+// TXAA (postProcess_TXAA.frag:171-270)
+// ---------------------------------------------------------------------------
+__device__ __forceinline__ float4 unorm4(uint32_t t) {
+  const float k = MTC_INV255;
+  return make_float4((float)(t & 0xffu) * k, (float)((t >> 8) & 0xffu) * k, (float)((t >> 16) & 0xffu) * k,
+                     (float)(t >> 24) * k);
+}
+__device__ __forceinline__ float4 f4min(float4 a, float4 b) {
+  return make_float4(fminf(a.x, b.x), fminf(a.y, b.y), fminf(a.z, b.z), fminf(a.w, b.w));
+}
+__device__ __forceinline__ float4 f4max(float4 a, float4 b) {
+  return make_float4(fmaxf(a.x, b.x), fmaxf(a.y, b.y), fmaxf(a.z, b.z), fmaxf(a.w, b.w));
+}
+__device__ __forceinline__ float4 f4add(float4 a, float4 b) {
+  return make_float4(a.x + b.x, a.y + b.y, a.z + b.z, a.w + b.w);
+}
+__device__ __forceinline__ float4 f4scale(float4 a, float s) { return make_float4(a.x * s, a.y * s, a.z * s, a.w * s); }
+__device__ __forceinline__ float4 f4div(float4 a, float s) { return make_float4(a.x / s, a.y / s, a.z / s, a.w / s); }
+
+// imageLoad of the current RGBA8 frame; outside the image it returns 0
+__device__ __forceinline__ float4 load_cur(const uint32_t* __restrict__ img, int W, int H, int x, int y) {
+  if (x < 0 || y < 0 || x >= W || y >= H) return make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+  return unorm4(img[(size_t)y * W + x]);
+}
+
+// texture(prevFrameImage, uv): RGBA8 codes filtered, then scaled by 1/255;
+// CLAMP_TO_BORDER with opaque black
+__device__ __forceinline__ float4 sample_prev(const uint32_t* __restrict__ img, int W, int H, float u, float v) {
+  const float x = u * (float)W - 0.5f, y = v * (float)H - 0.5f;
+  const float fx = floorf(x), fy = floorf(y);
+  const float wx = x - fx, wy = y - fy;
+  const int ix = (int)fminf(fmaxf(fx, -2.0f), (float)W), iy = (int)fminf(fmaxf(fy, -2.0f), (float)H);
+  c4 t[4];
+#pragma unroll
+  for (int k = 0; k < 4; ++k) {
+    const int xx = ix + (k & 1), yy = iy + (k >> 1);
+    t[k] = (xx < 0 || yy < 0 || xx >= W || yy >= H) ? c4{0.0f, 0.0f, 0.0f, 255.0f}
+                                                    : unpack8(img[(size_t)yy * W + xx]);
+  }
+  c4 r = scale4(lerp4(lerp4(t[0], t[1], wx), lerp4(t[2], t[3], wx), wy), MTC_INV255);
+  return make_float4(r.r, r.g, r.b, r.a);
+}
+
+__global__ __launch_bounds__(256) void txaa_kernel(TxaaConsts tc, const uint32_t* __restrict__ cur,
+                                                   const uint32_t* __restrict__ prev, uint32_t* __restrict__ out) {
+  const int W = tc.width, H = tc.height;
+  const int px = blockIdx.x * 64 + (threadIdx.x & 63);
+  const int py = blockIdx.y * 4 + (threadIdx.x >> 6);
+  if (px >= W || py >= H) return;
+  const float u = ((float)px + 0.5f) / (float)W, v = ((float)py + 0.5f) / (float)H;
+  // castRay (:87-116) with the basis folded per frame
+  float nx = u * 2.0f - 1.0f, ny = v * 2.0f - 1.0f;
+  nx = nx + tc.jit[0];
+  ny = ny + tc.jit[1];
+  const f3 eye = ld3(tc.eye);
+  const f3 cx = (nx * tc.tfx) * ld3(tc.right), cy = (ny * tc.tfy) * ld3(tc.up);
+  const f3 pnt = (ld3(tc.ref) + cx) + cy;
+  const f3 rD = normalize(pnt - eye);
+  // raySphereIntersection (:119-165) with the inner atmosphere of the TXAA shader
+  const f3 rO = ld3(tc.ro);
+  f3 P = mk(0.0f, 0.0f, 0.0f);
+  {
+    const float A = dot(rD, rD), B = 2.0f * dot(rD, rO), C = tc.C;
+    const float disc = B * B - 4.0f * A * C;
+    if (!(disc < 0.0f)) {
+      float t = (-B - __builtin_sqrtf(disc)) / (2.0f * A);
+      if (t < 0.0f) t = (-B + __builtin_sqrtf(disc)) / (2.0f * A);
+      if (t >= 0.0f) P = ((rO + t * rD) * tc.R) + ld3(tc.center);
+    }
+  }
+  const float* m = tc.old_view;
+  f3 q = mk(((m[0] * P.x + m[4] * P.y) + m[8] * P.z) + m[12] * 1.0f,
+            ((m[1] * P.x + m[5] * P.y) + m[9] * P.z) + m[13] * 1.0f,
+            ((m[2] * P.x + m[6] * P.y) + m[10] * P.z) + m[14] * 1.0f);
+  f3 od = normalize(q);
+  const float nz = -od.z;
+  od = od / nz;
+  const float ou = (od.x / tc.tfx) * 0.5f + 0.5f, ov = (od.y / tc.tfy) * 0.5f + 0.5f;
+  // neighbourHoodClamping (:191-220): 3x3 and cross-5 min / max / mean
+  const float4 ctl = load_cur(cur, W, H, px - 1, py - 1), ctc = load_cur(cur, W, H, px, py - 1),
+               ctr = load_cur(cur, W, H, px + 1, py - 1), cml = load_cur(cur, W, H, px - 1, py),
+               cmc = load_cur(cur, W, H, px, py), cmr = load_cur(cur, W, H, px + 1, py),
+               cbl = load_cur(cur, W, H, px - 1, py + 1), cbc = load_cur(cur, W, H, px, py + 1),
+               cbr = load_cur(cur, W, H, px + 1, py + 1);
+  float4 cmin = f4min(ctl, f4min(ctc, f4min(ctr, f4min(cml, f4min(cmc, f4min(cmr, f4min(cbl, f4min(cbc, cbr))))))));
+  float4 cmax = f4max(ctl, f4max(ctc, f4max(ctr, f4max(cml, f4max(cmc, f4max(cmr, f4max(cbl, f4max(cbc, cbr))))))));
+  float4 cavg = f4div(f4add(f4add(f4add(f4add(f4add(f4add(f4add(f4add(ctl, ctc), ctr), cml), cmc), cmr), cbl), cbc),
+                            cbr),
+                      9.0f);
+  const float4 cmin5 = f4min(ctc, f4min(cml, f4min(cmc, f4min(cmr, cbc))));
+  const float4 cmax5 = f4max(ctc, f4max(cml, f4max(cmc, f4max(cmr, cbc))));
+  const float4 cavg5 = f4div(f4add(f4add(f4add(f4add(ctc, cml), cmc), cmr), cbc), 5.0f);
+  cmin = f4scale(f4add(cmin, cmin5), 0.5f);
+  cmax = f4scale(f4add(cmax, cmax5), 0.5f);
+  cavg = f4scale(f4add(cavg, cavg5), 0.5f);
+  const float4 curr = cmc;
+  float4 prv = sample_prev(prev, W, H, ou, ov);
+  // clip_aabb(cmin.xyz, cmax.xyz, clamp(cavg, cmin, cmax), prev) (:170-189)
+  {
+    const float pw = fminf(fmaxf(cavg.w, cmin.w), cmax.w);
+    const float pcx = 0.5f * (cmax.x + cmin.x), pcy = 0.5f * (cmax.y + cmin.y), pcz = 0.5f * (cmax.z + cmin.z);
+    const float ex = 0.5f * (cmax.x - cmin.x) + 1e-10f, ey = 0.5f * (cmax.y - cmin.y) + 1e-10f,
+                ez = 0.5f * (cmax.z - cmin.z) + 1e-10f;
+    const float vx = prv.x - pcx, vy = prv.y - pcy, vz = prv.z - pcz, vw = prv.w - pw;
+    const float ma = fmaxf(fabsf(vx / ex), fmaxf(fabsf(vy / ey), fabsf(vz / ez)));
+    if (ma > 1.0f) prv = make_float4(pcx + vx / ma, pcy + vy / ma, pcz + vz / ma, pw + vw / ma);
+  }
+  // luminance-weighted feedback (:250-258)
+  const float lum0 = (curr.x * 0.2125f + curr.y * 0.7154f) + curr.z * 0.0721f;
+  const float lum1 = (prv.x * 0.2125f + prv.y * 0.7154f) + prv.z * 0.0721f;
+  const float diff = fabsf(lum0 - lum1) / fmaxf(lum0, fmaxf(lum1, 0.2f));
+  const float w = 1.0f - diff;
+  const float k = mixf(0.0f, 0.5f, w * w);
+  const float4 o = make_float4(mixf(prv.x, curr.x, k), mixf(prv.y, curr.y, k), mixf(prv.z, curr.z, k),
+                               mixf(prv.w, curr.w, k));
+  out[(size_t)py * W + px] = to_unorm8(o.x) | (to_unorm8(o.y) << 8) | (to_unorm8(o.z) << 16) | (to_unorm8(o.w) << 24);
+}
+
 }  // namespace post
 }  // namespace mtc
 
@@ -159,5 +285,11 @@ hipError_t launch_tone_map(const float4* img, int W, int H, uint32_t seed_s, flo
   dim3 grid((W + 63) / 64, (H + 3) / 4), block(256);
   if (f32) hipLaunchKernelGGL(mtc::post::tone_map_kernel<true>, grid, block, 0, s, img, W, H, seed_s, whitemap, out);
   else hipLaunchKernelGGL(mtc::post::tone_map_kernel<false>, grid, block, 0, s, img, W, H, seed_s, whitemap, out);
+  return hipGetLastError();
+}
+
+hipError_t launch_txaa(const TxaaConsts& tc, const uint32_t* cur, const uint32_t* prev, uint32_t* out, hipStream_t s) {
+  dim3 grid((tc.width + 63) / 64, (tc.height + 3) / 4), block(256);
+  hipLaunchKernelGGL(mtc::post::txaa_kernel, grid, block, 0, s, tc, cur, prev, out);
   return hipGetLastError();
 }

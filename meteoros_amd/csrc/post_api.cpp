@@ -4,11 +4,13 @@
 //                  reference's command buffer, Renderer.cpp:826-832; run
 //                  here as a compute pass on the latest result image)
 //   mtc_tone_map   postProcess_ToneMap.frag:68-84 (Renderer.cpp:834-838)
+//   mtc_txaa       postProcess_TXAA.frag:222-270 (Renderer.cpp:840-846)
 // The host folds the per-frame constants in FP32 with GLSL's operation
 // order (compiled with -ffp-contract=off).
 #include <hip/hip_runtime.h>
 
 #include <cmath>
+#include <cstring>
 #include <string>
 
 #include "runtime_internal.h"
@@ -16,6 +18,7 @@
 hipError_t launch_god_rays(const PostConsts& pc, const float4* mask, float4* footprint, float4* img, hipStream_t s);
 hipError_t launch_tone_map(const float4* img, int W, int H, uint32_t seed_s, float whitemap, int f32, void* out,
                            hipStream_t s);
+hipError_t launch_txaa(const TxaaConsts& tc, const uint32_t* cur, const uint32_t* prev, uint32_t* out, hipStream_t s);
 
 namespace {
 
@@ -40,6 +43,22 @@ float norm3(float v[3]) {
   v[1] = v[1] / l;
   v[2] = v[2] / l;
   return l;
+}
+
+// The two RGBA8 TXAA buffers, allocated and cleared (black history) on
+// first use.
+hipError_t ensure_txaa(mtc_ctx* c, hipStream_t s) {
+  if (c->txaa[0] && c->txaa[1]) return hipSuccess;
+  const size_t bytes = (size_t)c->desc.width * c->desc.height * 4;
+  for (int k = 0; k < 2; ++k) {
+    if (!c->txaa[k]) {
+      hipError_t e = hipMalloc((void**)&c->txaa[k], bytes);
+      if (e == hipSuccess) e = hipMemsetAsync(c->txaa[k], 0, bytes, s);
+      if (e != hipSuccess) return e;
+    }
+  }
+  c->txaa_latest = 0;
+  return hipStreamSynchronize(s);
 }
 
 // (A x + C B) ... of postProcess_ToneMap.frag:32-35 in FP32
@@ -86,6 +105,45 @@ bool god_ray_consts(const mtc_camera* cam, const mtc_sun* sun, int W, int H, Pos
   return !(pc->blend < 0.0f);
 }
 
+// TXAA constants (postProcess_TXAA.frag:87-116, :225-246): castRay's basis,
+// the jitter of getJitterOffset(frameCountMod16, dim) (:58-76, the march's
+// table), and the inner atmosphere of the TXAA shader, EARTH_RADIUS + 7500
+// with the earth centre below the eye (:229-233).
+void txaa_consts(const mtc_camera* cur, const mtc_camera* old, const mtc_time* tm, int W, int H, TxaaConsts* tc) {
+  const float eye[3] = {-cur->eye[0], -cur->eye[1], -cur->eye[2]};
+  const float* v = cur->view;
+  float right[3] = {v[0], v[4], v[8]}, up[3] = {v[1], v[5], v[9]}, lk[3] = {v[2], v[6], v[10]};
+  norm3(right);
+  norm3(up);
+  norm3(lk);
+  for (int k = 0; k < 3; ++k) {
+    tc->eye[k] = eye[k];
+    tc->right[k] = right[k];
+    tc->up[k] = up[k];
+    tc->ref[k] = eye[k] + (-lk[k]);
+  }
+  const int idx = tm->frame_count_mod16 / 2;
+  const float jx = idx < 4 ? tm->halton1[idx] : tm->halton3[idx - 4];
+  const float jy = idx < 4 ? tm->halton2[idx] : tm->halton4[idx - 4];
+  tc->jit[0] = jx / (float)W;
+  tc->jit[1] = jy / (float)H;
+  tc->tfx = cur->tan_fov_by2[0];
+  tc->tfy = cur->tan_fov_by2[1];
+  std::memcpy(tc->old_view, old->view, sizeof(tc->old_view));
+  const float R = 6371000.0f + 7500.0f;
+  const float center[3] = {eye[0], -6371000.0f, eye[2]};
+  float ro[3];
+  for (int k = 0; k < 3; ++k) {
+    ro[k] = (eye[k] - center[k]) / R;
+    tc->ro[k] = ro[k];
+    tc->center[k] = center[k];
+  }
+  tc->C = ((ro[0] * ro[0] + ro[1] * ro[1]) + ro[2] * ro[2]) - 1.0f;
+  tc->R = R;
+  tc->width = W;
+  tc->height = H;
+}
+
 extern "C" {
 
 int mtc_god_rays(mtc_ctx* c, const mtc_camera* cam, const mtc_sun* sun, void* stream) {
@@ -127,6 +185,50 @@ int mtc_tone_map(mtc_ctx* c, const mtc_time* time, int32_t format, void* stream)
   hipStream_t s = stream ? (hipStream_t)stream : c->stream;
   POST_OK(launch_tone_map(c->img[c->latest], W, H, seed_s, whitemap, format == MTC_DISPLAY_RGBA32F, c->display, s),
           "mtc_tone_map: launch");
+  return MTC_OK;
+}
+
+int mtc_txaa(mtc_ctx* c, const mtc_camera* cur, const mtc_camera* old, const mtc_time* time, void* stream) {
+  if (!c || !cur || !old || !time) return mtc_fail(MTC_ERR_INVALID_ARG, "mtc_txaa: null pointer");
+  if (!c->display || c->display_format != MTC_DISPLAY_RGBA8)
+    return mtc_fail(MTC_ERR_STATE, "mtc_txaa: needs an RGBA8 tone-mapped image (mtc_tone_map) first");
+  if (!finite_all(cur->view, 16) || !finite_all(cur->eye, 4) || !finite_all(cur->tan_fov_by2, 2) ||
+      !finite_all(old->view, 16))
+    return mtc_fail(MTC_ERR_INVALID_ARG, "mtc_txaa: non-finite camera");
+  if (time->frame_count_mod16 < 0 || time->frame_count_mod16 > 15 || !finite_all(time->halton1, 4) ||
+      !finite_all(time->halton2, 4) || !finite_all(time->halton3, 4) || !finite_all(time->halton4, 4))
+    return mtc_fail(MTC_ERR_INVALID_ARG, "mtc_txaa: bad time uniform");
+  if (hipSetDevice(c->desc.device) != hipSuccess) return mtc_fail(MTC_ERR_HIP, "mtc_txaa: hipSetDevice failed");
+  const int W = c->desc.width, H = c->desc.height;
+  hipStream_t s = stream ? (hipStream_t)stream : c->stream;
+  POST_OK(ensure_txaa(c, s), "mtc_txaa: history");
+  TxaaConsts tc;
+  txaa_consts(cur, old, time, W, H, &tc);
+  const int dst = 1 - c->txaa_latest;
+  POST_OK(launch_txaa(tc, (const uint32_t*)c->display, c->txaa[c->txaa_latest], c->txaa[dst], s), "mtc_txaa: launch");
+  c->txaa_latest = dst;
+  return MTC_OK;
+}
+
+int mtc_read_txaa(mtc_ctx* c, uint8_t* rgba8, size_t n_bytes) {
+  if (!c || !rgba8) return mtc_fail(MTC_ERR_INVALID_ARG, "mtc_read_txaa: null pointer");
+  if (!c->txaa[0]) return mtc_fail(MTC_ERR_STATE, "mtc_read_txaa: no TXAA output yet");
+  const size_t need = (size_t)c->desc.width * c->desc.height * 4;
+  if (n_bytes < need) return mtc_fail(MTC_ERR_INVALID_ARG, "mtc_read_txaa: buffer too small");
+  if (hipSetDevice(c->desc.device) != hipSuccess) return mtc_fail(MTC_ERR_HIP, "mtc_read_txaa: hipSetDevice failed");
+  POST_OK(hipDeviceSynchronize(), "mtc_read_txaa: sync");
+  POST_OK(hipMemcpy(rgba8, c->txaa[c->txaa_latest], need, hipMemcpyDeviceToHost), "mtc_read_txaa: copy");
+  return MTC_OK;
+}
+
+int mtc_set_txaa_history(mtc_ctx* c, const uint8_t* rgba8, size_t n_bytes) {
+  if (!c || !rgba8) return mtc_fail(MTC_ERR_INVALID_ARG, "mtc_set_txaa_history: null pointer");
+  const size_t need = (size_t)c->desc.width * c->desc.height * 4;
+  if (n_bytes != need) return mtc_fail(MTC_ERR_INVALID_ARG, "mtc_set_txaa_history: need exactly W*H*4 bytes");
+  if (hipSetDevice(c->desc.device) != hipSuccess) return mtc_fail(MTC_ERR_HIP, "mtc_set_txaa_history: hipSetDevice failed");
+  POST_OK(hipDeviceSynchronize(), "mtc_set_txaa_history: sync");
+  POST_OK(ensure_txaa(c, c->stream), "mtc_set_txaa_history: alloc");
+  POST_OK(hipMemcpy(c->txaa[c->txaa_latest], rgba8, need, hipMemcpyHostToDevice), "mtc_set_txaa_history: copy");
   return MTC_OK;
 }
 

@@ -55,12 +55,16 @@ __device__ __forceinline__ f3 inner_hit(const FrameConsts& fc, f3 rD, f3 c, floa
   return point;
 }
 
-template <bool INDICES>
+// SKIP_SELECTED: leave out the pixels this frame's march overwrites (the
+// 4x4 phase (frame / 4, frame % 4)), so the two kernels write disjoint
+// pixels and may run concurrently.
+template <bool INDICES, bool SKIP_SELECTED>
 __global__ __launch_bounds__(256) void reproject_kernel(FrameConsts fc, OutputViews ov) {
   const int x = blockIdx.x * 64 + (threadIdx.x & 63);
   const int y = blockIdx.y * 4 + (threadIdx.x >> 6);
   const int W = fc.width, H = fc.height;
   if (x >= W || y >= H) return;
+  if (SKIP_SELECTED && (x & 3) == fc.frame / 4 && (y & 3) == fc.frame % 4) return;
   float ux = (float)x / (float)W, uy = (float)y / (float)H;
   Ray ray = cast_ray(fc, ux, uy);
   f3 c = ray.o;
@@ -110,11 +114,13 @@ __global__ __launch_bounds__(256) void reproject_kernel(FrameConsts fc, OutputVi
 }  // namespace reproj
 }  // namespace mtc
 
-// mode 0: the update; 1: the tap-index map
+// mode 0: the update; 1: the tap-index map; 2: the update without the
+// pixels the march of this frame writes
 hipError_t launch_reproject(const FrameConsts& fc, const OutputViews& ov, int mode, hipStream_t stream) {
   dim3 grid((fc.width + 63) / 64, (fc.height + 3) / 4);
   dim3 block(256);
-  if (mode == 1) hipLaunchKernelGGL((mtc::reproj::reproject_kernel<true>), grid, block, 0, stream, fc, ov);
-  else hipLaunchKernelGGL((mtc::reproj::reproject_kernel<false>), grid, block, 0, stream, fc, ov);
+  if (mode == 1) hipLaunchKernelGGL((mtc::reproj::reproject_kernel<true, false>), grid, block, 0, stream, fc, ov);
+  else if (mode == 2) hipLaunchKernelGGL((mtc::reproj::reproject_kernel<false, true>), grid, block, 0, stream, fc, ov);
+  else hipLaunchKernelGGL((mtc::reproj::reproject_kernel<false, false>), grid, block, 0, stream, fc, ov);
   return hipGetLastError();
 }

@@ -119,6 +119,11 @@ void release(mtc_ctx* c) {
   (void)hipFree(c->q_weather);
   (void)hipFree(c->ray_fp);
   (void)hipFree(c->display);
+  (void)hipFree(c->txaa[0]);
+  (void)hipFree(c->txaa[1]);
+  if (c->fork) (void)hipEventDestroy(c->fork);
+  if (c->join) (void)hipEventDestroy(c->join);
+  if (c->aux) (void)hipStreamDestroy(c->aux);
   if (c->stream) (void)hipStreamDestroy(c->stream);
   delete c;
 }
@@ -236,6 +241,9 @@ int mtc_create(const mtc_desc* d, mtc_ctx** out) {
   if (e == hipSuccess) e = hipMalloc((void**)&c->mask, px * sizeof(float4));
   if (e == hipSuccess) e = hipMalloc((void**)&c->counters, 16 * sizeof(unsigned long long));
   if (d->kernel != MTC_KERNEL_PLAIN) {
+    if (e == hipSuccess) e = hipStreamCreateWithFlags(&c->aux, hipStreamNonBlocking);
+    if (e == hipSuccess) e = hipEventCreateWithFlags(&c->fork, hipEventDisableTiming);
+    if (e == hipSuccess) e = hipEventCreateWithFlags(&c->join, hipEventDisableTiming);
     const size_t lf3 = (size_t)v.lf_dim * v.lf_dim * v.lf_dim, hf3 = (size_t)v.hf_dim * v.hf_dim * v.hf_dim;
     if (e == hipSuccess) e = hipMalloc((void**)&c->q_lf, lf3 * sizeof(uint4));
     if (e == hipSuccess) e = hipMalloc((void**)&c->q_hf, hf3 * sizeof(uint4));
@@ -337,6 +345,18 @@ int mtc_frame(mtc_ctx* c, const mtc_camera* cur, const mtc_camera* old, const mt
       ev[k] = take_event(c);
       if (!ev[k]) return mtc_fail(MTC_ERR_HIP, "mtc_frame: hipEventCreate failed");
     }
+  }
+  if (!full && !timing && c->aux) {
+    // reprojection of the other 15/16 of the pixels beside the march
+    HIP_TRY(hipEventRecord(c->fork, s), "mtc_frame: fork");
+    HIP_TRY(hipStreamWaitEvent(c->aux, c->fork, 0), "mtc_frame: fork");
+    HIP_TRY(launch_reproject(fc, ov, 2, c->aux), "reprojection kernel launch");
+    HIP_TRY(hipEventRecord(c->join, c->aux), "mtc_frame: join");
+    rc = launch_march(c, fc, ov, full, s);
+    if (rc) return rc;
+    HIP_TRY(hipStreamWaitEvent(s, c->join, 0), "mtc_frame: join");
+    c->latest = dst;
+    return MTC_OK;
   }
   if (!full) {
     if (timing) HIP_TRY(hipEventRecord(ev[0], s), "mtc_frame: event");

@@ -101,6 +101,22 @@ struct PostConsts {
   int32_t width, height;
 };
 
+// Per-frame constants of the TXAA pass (postProcess_TXAA.frag:228-246),
+// folded on the host in FP32 with the shader's operation order.
+struct TxaaConsts {
+  float eye[3];           // eyePos = -camera.eye
+  float right[3], up[3];  // normalize(view row 0), normalize(view row 1)
+  float ref[3];           // eye + look
+  float jit[2];           // getJitterOffset(frameCountMod16, dim)
+  float tfx, tfy;         // camera.tanFovBy2
+  float old_view[16];     // cameraOld.view
+  float ro[3];            // (eye - earthCenter) / ATMOSPHERE_RADIUS_INNER
+  float C;                // dot(ro, ro) - 1
+  float center[3];        // earthCenter = (eye.x, -EARTH_RADIUS, eye.z)
+  float R;                // ATMOSPHERE_RADIUS_INNER
+  int32_t width, height;
+};
+
 // Learned-light network as a frame kernel sees it.
 struct LearnedLight {
   const void* frag;       // packed BF16 MFMA weight fragments
@@ -115,6 +131,10 @@ struct LearnedLight {
 struct mtc_ctx {
   mtc_desc desc;
   hipStream_t stream = nullptr;
+  // optimised kernels: the reprojection runs on `aux` beside the march
+  // (fork / join events), the two writing disjoint pixels
+  hipStream_t aux = nullptr;
+  hipEvent_t fork = nullptr, join = nullptr;
   void* lf = nullptr;
   void* hf = nullptr;
   uint32_t* curl = nullptr;
@@ -140,6 +160,9 @@ struct mtc_ctx {
   float4* ray_fp = nullptr;
   void* display = nullptr;
   int display_format = -1;
+  // TXAA: RGBA8 history / output ping-pong, txaa_latest = the last output
+  uint32_t* txaa[2] = {nullptr, nullptr};
+  int txaa_latest = 0;
   // MTC_KERNEL_LEARNED: attached network (not owned) and its audit period
   mtc_mlp* mlp = nullptr;
   int mlp_audit = 0;

@@ -226,6 +226,18 @@ class CloudStage:
         check(lib().mtc_read_display(self._ctx, out.ctypes.data, out.nbytes), "mtc_read_display")
         return out
 
+    def txaa(self, cam: A.MtcCamera, old: A.MtcCamera, time: A.MtcTime, stream: int | None = None) -> None:
+        check(lib().mtc_txaa(self._ctx, C.byref(cam), C.byref(old), C.byref(time), stream), "mtc_txaa")
+
+    def read_txaa(self) -> np.ndarray:
+        out = np.empty((self.height, self.width, 4), dtype=np.uint8)
+        check(lib().mtc_read_txaa(self._ctx, out.ctypes.data, out.nbytes), "mtc_read_txaa")
+        return out
+
+    def set_txaa_history(self, rgba8: np.ndarray) -> None:
+        a = np.ascontiguousarray(rgba8, dtype=np.uint8)
+        check(lib().mtc_set_txaa_history(self._ctx, a.ctypes.data, a.nbytes), "mtc_set_txaa_history")
+
     def reprojection_indices(self, cam, old, time, params) -> np.ndarray:
         n = self.width * self.height * int(params.motion_blur_samples)
         idx = np.empty(n, dtype=np.int32)

@@ -46,6 +46,9 @@ def lib() -> C.CDLL:
         h.orc_god_rays.argtypes = [P(A.MtcCamera), P(A.MtcSun), C.c_int, C.c_int, vp, vp, vp, C.c_int, vp]
         h.orc_tone_map.restype = C.c_int
         h.orc_tone_map.argtypes = [C.c_uint32, C.c_int, C.c_int, vp, vp, vp, C.c_int]
+        h.orc_txaa.restype = C.c_int
+        h.orc_txaa.argtypes = [P(A.MtcCamera), P(A.MtcCamera), P(A.MtcTime), C.c_int, C.c_int, vp, vp, vp, vp, vp,
+                               C.c_int]
         h.orc_wang_hash.restype = C.c_uint32
         h.orc_wang_hash.argtypes = [C.c_uint32, C.c_uint32, C.c_uint32]
         h.orc_uncharted2.restype = C.c_float
@@ -198,6 +201,21 @@ def tone_map(time_s: int, rgba: np.ndarray, precision=32):
     if lib().orc_tone_map(int(time_s), W, H, a.ctypes.data, rgb.ctypes.data, out8.ctypes.data, precision):
         raise ValueError("orc_tone_map rejected its arguments")
     return rgb, out8
+
+
+def txaa(cam, old, time, cur8: np.ndarray, prev8: np.ndarray, precision=32):
+    """postProcess_TXAA.frag over HxWx4 RGBA8 current and history images.
+    Returns (blend before quantisation [H,W,4], RGBA8 [H,W,4], old uv [H,W,2])."""
+    H, W = cur8.shape[:2]
+    c = np.ascontiguousarray(cur8, dtype=np.uint8)
+    p = np.ascontiguousarray(prev8, dtype=np.uint8)
+    rgba = np.empty((H, W, 4), dtype=np.float64)
+    out8 = np.empty((H, W, 4), dtype=np.uint8)
+    uv = np.empty((H, W, 2), dtype=np.float64)
+    if lib().orc_txaa(C.byref(cam), C.byref(old), C.byref(time), W, H, c.ctypes.data, p.ctypes.data,
+                      rgba.ctypes.data, out8.ctypes.data, uv.ctypes.data, precision):
+        raise ValueError("orc_txaa rejected its arguments")
+    return rgba, out8, uv
 
 
 def wang_hash(u: int, v: int, s: int) -> int:

@@ -398,6 +398,38 @@ int orc_tone_map(uint32_t time_s, int W, int H, const float* rgba, double* rgb, 
   return 0;
 }
 
+// TXAA of postProcess_TXAA.frag: cur and prev are W x H RGBA8; rgba (W*H*4
+// doubles, may be NULL) receives the blend before quantisation, out8 the
+// RGBA8 store, old_uv (W*H*2, may be NULL) the reprojected uv.
+int orc_txaa(const mtc_camera* cam, const mtc_camera* old, const mtc_time* tm, int W, int H, const uint8_t* cur,
+             const uint8_t* prev, double* rgba, uint8_t* out8, double* old_uv, int precision) {
+  if (!cam || !old || !tm || !cur || !prev || W <= 0 || H <= 0) return 1;
+#pragma omp parallel for schedule(static)
+  for (int py = 0; py < H; ++py)
+    for (int px = 0; px < W; ++px) {
+      double o[4], uv[2];
+      if (precision == 64) {
+        auto r = orc_post::txaa_pixel<double>(*cam, *old, *tm, cur, prev, W, H, px, py, uv);
+        o[0] = r.x; o[1] = r.y; o[2] = r.z; o[3] = r.w;
+      } else {
+        float uvf[2];
+        auto r = orc_post::txaa_pixel<float>(*cam, *old, *tm, cur, prev, W, H, px, py, uvf);
+        o[0] = r.x; o[1] = r.y; o[2] = r.z; o[3] = r.w;
+        uv[0] = uvf[0]; uv[1] = uvf[1];
+      }
+      const size_t i = (size_t)py * W + px;
+      for (int k = 0; k < 4; ++k) {
+        if (rgba) rgba[i * 4 + k] = o[k];
+        if (out8) out8[i * 4 + k] = (uint8_t)std::nearbyint(std::fmin(std::fmax((float)o[k], 0.0f), 1.0f) * 255.0f);
+      }
+      if (old_uv) {
+        old_uv[i * 2] = uv[0];
+        old_uv[i * 2 + 1] = uv[1];
+      }
+    }
+  return 0;
+}
+
 uint32_t orc_wang_hash(uint32_t u, uint32_t v, uint32_t s) { return orc_post::wang_hash(u, v, s); }
 float orc_uncharted2(float x) { return orc_post::uncharted2<float>(x); }
 

@@ -149,4 +149,134 @@ static void tone_map_pixel(const float* rgba, int W, int H, int px, int py, uint
   }
 }
 
+// ---------------------------------------------------------------------------
+// TXAA (postProcess_TXAA.frag:58-270)
+// ---------------------------------------------------------------------------
+template <class T> struct V3 { T x, y, z; };
+template <class T> static inline V3<T> v3n(T x, T y, T z) {
+  T l = std::sqrt(x * x + y * y + z * z);
+  return V3<T>{x / l, y / l, z / l};
+}
+
+// imageLoad of an RGBA8 image as UNORM; 0 outside
+template <class T> static inline P4<T> load8(const uint8_t* img, int W, int H, int x, int y) {
+  if (x < 0 || y < 0 || x >= W || y >= H) return P4<T>{T(0), T(0), T(0), T(0)};
+  const uint8_t* p = img + ((size_t)y * W + x) * 4;
+  const T k = (T)(float)(1.0 / 255.0);
+  return P4<T>{(T)p[0] * k, (T)p[1] * k, (T)p[2] * k, (T)p[3] * k};
+}
+// texture() of an RGBA8 image: codes filtered, then scaled; border (0, 0, 0, 1)
+template <class T> static inline P4<T> sample8(const uint8_t* img, int W, int H, T u, T v) {
+  T x = u * (T)W - T(0.5), y = v * (T)H - T(0.5);
+  T fx = std::floor(x), fy = std::floor(y);
+  T wx = x - fx, wy = y - fy;
+  int ix = (int)std::fmin(std::fmax(fx, T(-2)), (T)W), iy = (int)std::fmin(std::fmax(fy, T(-2)), (T)H);
+  P4<T> t[4];
+  for (int k = 0; k < 4; ++k) {
+    int xx = ix + (k & 1), yy = iy + (k >> 1);
+    if (xx < 0 || yy < 0 || xx >= W || yy >= H) {
+      t[k] = P4<T>{T(0), T(0), T(0), T(255)};
+    } else {
+      const uint8_t* p = img + ((size_t)yy * W + xx) * 4;
+      t[k] = P4<T>{(T)p[0], (T)p[1], (T)p[2], (T)p[3]};
+    }
+  }
+  P4<T> r = lerp4(lerp4(t[0], t[1], wx), lerp4(t[2], t[3], wx), wy);
+  const T k = (T)(float)(1.0 / 255.0);
+  return P4<T>{r.x * k, r.y * k, r.z * k, r.w * k};
+}
+template <class T> static inline P4<T> pmin(P4<T> a, P4<T> b) {
+  return P4<T>{std::fmin(a.x, b.x), std::fmin(a.y, b.y), std::fmin(a.z, b.z), std::fmin(a.w, b.w)};
+}
+template <class T> static inline P4<T> pmax(P4<T> a, P4<T> b) {
+  return P4<T>{std::fmax(a.x, b.x), std::fmax(a.y, b.y), std::fmax(a.z, b.z), std::fmax(a.w, b.w)};
+}
+template <class T> static inline P4<T> padd(P4<T> a, P4<T> b) { return P4<T>{a.x + b.x, a.y + b.y, a.z + b.z, a.w + b.w}; }
+template <class T> static inline P4<T> pmul(P4<T> a, T s) { return P4<T>{a.x * s, a.y * s, a.z * s, a.w * s}; }
+template <class T> static inline P4<T> pdiv(P4<T> a, T s) { return P4<T>{a.x / s, a.y / s, a.z / s, a.w / s}; }
+
+// main (:222-270) for one pixel; returns the blended colour before the
+// RGBA8 store.  Also reports the reprojected uv.
+template <class T>
+static P4<T> txaa_pixel(const mtc_camera& cam, const mtc_camera& old, const mtc_time& tm, const uint8_t* cur,
+                        const uint8_t* prev, int W, int H, int px, int py, T* old_uv) {
+  const T u = ((T)px + T(0.5)) / (T)W, v = ((T)py + T(0.5)) / (T)H;
+  const T ex = -(T)cam.eye[0], ey = -(T)cam.eye[1], ez = -(T)cam.eye[2];
+  const float* m = cam.view;
+  V3<T> right = v3n<T>(m[0], m[4], m[8]), up = v3n<T>(m[1], m[5], m[9]), lk = v3n<T>(m[2], m[6], m[10]);
+  // getJitterOffset(frameCountMod16, dim) (:58-76)
+  const int idx = tm.frame_count_mod16 / 2;
+  const T jx = (T)(idx < 4 ? tm.halton1[idx] : tm.halton3[idx - 4]) / (T)W;
+  const T jy = (T)(idx < 4 ? tm.halton2[idx] : tm.halton4[idx - 4]) / (T)H;
+  T nx = u * T(2) - T(1) + jx, ny = v * T(2) - T(1) + jy;
+  const T tfx = (T)cam.tan_fov_by2[0], tfy = (T)cam.tan_fov_by2[1];
+  const T cxs = nx * tfx, cys = ny * tfy;
+  const T p0 = ((ex - lk.x) + cxs * right.x) + cys * up.x, p1 = ((ey - lk.y) + cxs * right.y) + cys * up.y,
+          p2 = ((ez - lk.z) + cxs * right.z) + cys * up.z;
+  V3<T> d = v3n<T>(p0 - ex, p1 - ey, p2 - ez);
+  // raySphereIntersection with EARTH_RADIUS + 7500, centre (eye.x, -EARTH_RADIUS, eye.z) (:119-165)
+  const T R = (T)6371000.0f + (T)7500.0f;
+  const T c0 = ex, c1 = -(T)6371000.0f, c2 = ez;
+  const T r0 = (ex - c0) / R, r1 = (ey - c1) / R, r2 = (ez - c2) / R;
+  const T A = (d.x * d.x + d.y * d.y) + d.z * d.z, B = T(2) * ((d.x * r0 + d.y * r1) + d.z * r2);
+  const T C = ((r0 * r0 + r1 * r1) + r2 * r2) - T(1);
+  const T disc = B * B - T(4) * A * C;
+  T P[3] = {T(0), T(0), T(0)};
+  if (!(disc < T(0))) {
+    T t = (-B - std::sqrt(disc)) / (T(2) * A);
+    if (t < T(0)) t = (-B + std::sqrt(disc)) / (T(2) * A);
+    if (t >= T(0)) {
+      P[0] = (r0 + t * d.x) * R + c0;
+      P[1] = (r1 + t * d.y) * R + c1;
+      P[2] = (r2 + t * d.z) * R + c2;
+    }
+  }
+  const float* o = old.view;
+  T q[3];
+  for (int r = 0; r < 3; ++r) q[r] = (((T)o[r] * P[0] + (T)o[4 + r] * P[1]) + (T)o[8 + r] * P[2]) + (T)o[12 + r];
+  V3<T> od = v3n<T>(q[0], q[1], q[2]);
+  const T nz = -od.z;
+  const T ou = ((od.x / nz) / tfx) * T(0.5) + T(0.5), ov = ((od.y / nz) / tfy) * T(0.5) + T(0.5);
+  if (old_uv) {
+    old_uv[0] = ou;
+    old_uv[1] = ov;
+  }
+  // neighbourHoodClamping (:191-220)
+  P4<T> n[9];
+  for (int k = 0; k < 9; ++k) n[k] = load8<T>(cur, W, H, px + (k % 3) - 1, py + (k / 3) - 1);
+  P4<T> cmin = n[8], cmax = n[8];
+  for (int k = 7; k >= 0; --k) {
+    cmin = pmin(n[k], cmin);
+    cmax = pmax(n[k], cmax);
+  }
+  P4<T> cavg = n[0];
+  for (int k = 1; k < 9; ++k) cavg = padd(cavg, n[k]);
+  cavg = pdiv(cavg, T(9));
+  P4<T> cmin5 = pmin(n[1], pmin(n[3], pmin(n[4], pmin(n[5], n[7]))));
+  P4<T> cmax5 = pmax(n[1], pmax(n[3], pmax(n[4], pmax(n[5], n[7]))));
+  P4<T> cavg5 = pdiv(padd(padd(padd(padd(n[1], n[3]), n[4]), n[5]), n[7]), T(5));
+  cmin = pmul(padd(cmin, cmin5), T(0.5));
+  cmax = pmul(padd(cmax, cmax5), T(0.5));
+  cavg = pmul(padd(cavg, cavg5), T(0.5));
+  const P4<T> curr = n[4];
+  P4<T> pr = sample8<T>(prev, W, H, ou, ov);
+  // clip_aabb (:170-189)
+  const T pw = std::fmin(std::fmax(cavg.w, cmin.w), cmax.w);
+  const T eps = (T)1e-10f;
+  const T pcx = T(0.5) * (cmax.x + cmin.x), pcy = T(0.5) * (cmax.y + cmin.y), pcz = T(0.5) * (cmax.z + cmin.z);
+  const T ecx = T(0.5) * (cmax.x - cmin.x) + eps, ecy = T(0.5) * (cmax.y - cmin.y) + eps,
+          ecz = T(0.5) * (cmax.z - cmin.z) + eps;
+  const T vx = pr.x - pcx, vy = pr.y - pcy, vz = pr.z - pcz, vw = pr.w - pw;
+  const T ma = std::fmax(std::fabs(vx / ecx), std::fmax(std::fabs(vy / ecy), std::fabs(vz / ecz)));
+  if (ma > T(1)) pr = P4<T>{pcx + vx / ma, pcy + vy / ma, pcz + vz / ma, pw + vw / ma};
+  // Luminance and feedback (:246-258)
+  const T l0 = (curr.x * (T)0.2125f + curr.y * (T)0.7154f) + curr.z * (T)0.0721f;
+  const T l1 = (pr.x * (T)0.2125f + pr.y * (T)0.7154f) + pr.z * (T)0.0721f;
+  const T diff = std::fabs(l0 - l1) / std::fmax(l0, std::fmax(l1, (T)0.2f));
+  const T w = T(1) - diff;
+  const T k = T(0) * (T(1) - w * w) + T(0.5) * (w * w);
+  auto mix = [&](T a, T b) { return a * (T(1) - k) + b * k; };
+  return P4<T>{mix(pr.x, curr.x), mix(pr.y, curr.y), mix(pr.z, curr.z), mix(pr.w, curr.w)};
+}
+
 }  // namespace orc_post

@@ -1,6 +1,7 @@
 """The post-process passes on the GPU against the CPU oracle
 (oracle/post_oracle.hpp): god rays over a marched frame's result and mask,
-tone map + dither in RGBA32F (before quantisation) and RGBA8.
+tone map + dither in RGBA32F (before quantisation) and RGBA8, and TXAA over
+two frames of a moving camera (RGBA8, same one-code bar).
 
 Bars: god rays within 1e-5 of the frame's peak (the kernel decodes the mask
 before filtering, the shader after: the same linear map, FP32 rounding
@@ -93,6 +94,37 @@ def test_god_rays_then_tone_map_at_1080p():
     assert np.abs(outf[..., :3] - rgb).max() <= 1e-4
 
 
+@pytest.mark.parametrize("yaw", [0.0, 0.5, 3.0])
+def test_txaa_matches_oracle(yaw):
+    """Two frames: tone map -> TXAA (black history), camera yaws, tone map ->
+    TXAA against the first output; each TXAA output against the FP32 oracle."""
+    W, H = 480, 270
+    params, sun = M.preset_params(2)
+    camera = ground_camera(W, H, 25.0)
+    v = volumes(*DIMS)
+    with M.CloudStage(W, H, M.VolumeSpec(*DIMS), kernel=A.MTC_KERNEL_FAST) as st:
+        st.set_volumes(v.lf, v.hf, v.curl, v.weather)
+        history = np.zeros((H, W, 4), np.uint8)
+        old = camera.ubo()
+        for k in range(2):
+            camera.rotate_about_up(yaw)
+            cur = camera.ubo()
+            t = M.Time()
+            t.ubo.frame_count_mod16 = 4 + k
+            t.ubo.time[1] = 2.0 + k / 60.0
+            st.march_full(cur, t.ubo, sun, params)
+            st.tone_map(t.ubo)
+            disp = st.read_display()
+            st.txaa(cur, old, t.ubo)
+            got = st.read_txaa()
+            _, ref8, _ = O.txaa(cur, old, t.ubo, disp, history, precision=32)
+            d = np.abs(got.astype(np.int16) - ref8.astype(np.int16))
+            print(f"yaw {yaw} frame {k}: {int((d > 0).sum())} of {d.size} bytes differ, max {int(d.max())}")
+            assert d.max() <= 1 and (d > 0).mean() <= 1e-3
+            history = got
+            old = cur
+
+
 def test_post_error_paths():
     with M.CloudStage(32, 16, M.VolumeSpec(32, 16, 32, 64)) as st:
         with pytest.raises(M.MtcError):
@@ -102,3 +134,8 @@ def test_post_error_paths():
         with pytest.raises(M.MtcError):
             st.tone_map(t)
         assert M.lib().mtc_tone_map(st.handle, M.Time().ubo, 7, None) == A.MTC_ERR_INVALID_ARG
+        cam = ground_camera(32, 16).ubo()
+        with pytest.raises(M.MtcError):
+            st.txaa(cam, cam, M.Time().ubo)  # no RGBA8 tone-mapped image yet
+        with pytest.raises(M.MtcError):
+            st.read_txaa()

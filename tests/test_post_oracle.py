@@ -177,3 +177,82 @@ def test_wang_hash_is_invertible_step_by_step():
     counts = np.bincount((hs >> np.uint64(28)).astype(np.int64), minlength=16)
     exp = len(hs) / 16
     assert ((counts - exp) ** 2 / exp).sum() < 45
+
+
+# --- TXAA (postProcess_TXAA.frag) ---------------------------------------------
+def _time_frame(frame):
+    t = M.Time()
+    t.ubo.frame_count_mod16 = frame
+    return t.ubo
+
+
+def _jitter(tm, frame, W, H):
+    idx = frame // 2
+    jx = tm.halton1[idx] if idx < 4 else tm.halton3[idx - 4]
+    jy = tm.halton2[idx] if idx < 4 else tm.halton4[idx - 4]
+    return jx / W, jy / H
+
+
+@pytest.mark.parametrize("frame", [0, 5, 11])
+def test_txaa_reprojection_with_a_still_camera_is_the_jitter(frame):
+    # same camera: the hit point projects back along its own jittered ray,
+    # so old_uv = uv + jitter / 2 (NDC = 2 uv - 1 + jitter)
+    W, H = 48, 32
+    cam = _camera(W, H, 25.0, 10.0)
+    tm = _time_frame(frame)
+    img = np.zeros((H, W, 4), np.uint8)
+    _, _, uv = O.txaa(cam, cam, tm, img, img, precision=64)
+    jx, jy = _jitter(tm, frame, W, H)
+    px, py = np.meshgrid(np.arange(W), np.arange(H))
+    assert np.allclose(uv[..., 0], (px + 0.5) / W + jx / 2, atol=1e-9)
+    assert np.allclose(uv[..., 1], (py + 0.5) / H + jy / 2, atol=1e-9)
+
+
+def test_txaa_constant_frame_is_a_fixed_point():
+    # a constant current frame collapses the clip box onto it: the output is
+    # the current colour whatever the history holds (interior pixels)
+    W, H = 24, 16
+    cam = _camera(W, H, 30.0)
+    cur = np.zeros((H, W, 4), np.uint8)
+    cur[...] = (90, 140, 200, 255)
+    rng = np.random.default_rng(5)
+    prev = rng.integers(0, 256, (H, W, 4), dtype=np.uint8)
+    _, out8, _ = O.txaa(cam, cam, _time_frame(3), cur, prev, precision=64)
+    assert np.array_equal(out8[1:-1, 1:-1], cur[1:-1, 1:-1])
+
+
+def test_txaa_feedback_weight_closed_form():
+    # checkerboard current frame (box = [0, 1] per channel, so no clipping)
+    # and a constant history P: out = mix(P, cur, 0.5 (1 - |l0 - l1| / max(l0, l1, 0.2))^2)
+    W, H = 20, 14
+    cam = _camera(W, H, 30.0)
+    ys, xs = np.mgrid[0:H, 0:W]
+    cur = np.zeros((H, W, 4), np.uint8)
+    cur[(xs + ys) % 2 == 0] = (255, 255, 255, 255)
+    cur[(xs + ys) % 2 == 1] = (0, 0, 0, 255)
+    prev = np.zeros((H, W, 4), np.uint8)
+    prev[...] = (100, 120, 140, 255)
+    rgba, _, _ = O.txaa(cam, cam, _time_frame(0), cur, prev, precision=64)
+    c = cur.astype(np.float64) / 255.0
+    p = prev.astype(np.float64) / 255.0
+    lum = lambda a: a[..., 0] * 0.2125 + a[..., 1] * 0.7154 + a[..., 2] * 0.0721  # noqa: E731
+    l0, l1 = lum(c), lum(p)
+    k = 0.5 * (1.0 - np.abs(l0 - l1) / np.maximum(l0, np.maximum(l1, 0.2))) ** 2
+    exp = p * (1 - k[..., None]) + c * k[..., None]
+    inner = (slice(2, -2), slice(2, -2))
+    assert np.allclose(rgba[inner], exp[inner], atol=1e-6)
+
+
+def test_txaa_twins_agree():
+    W, H = 40, 30
+    cam = _camera(W, H, 25.0)
+    c2 = M.Camera((0.0, 0.0, 0.0), (0.05, 0.42, -0.9), W, H, 45.0)
+    rng = np.random.default_rng(8)
+    cur = rng.integers(0, 256, (H, W, 4), dtype=np.uint8)
+    prev = rng.integers(0, 256, (H, W, 4), dtype=np.uint8)
+    a, a8, _ = O.txaa(cam, c2.ubo(), _time_frame(7), cur, prev, precision=64)
+    b, b8, _ = O.txaa(cam, c2.ubo(), _time_frame(7), cur, prev, precision=32)
+    # the FP32 shell hit (radius 6.4e6 m) moves the reprojected uv by a
+    # fraction of a texel; on a random history that is visible
+    assert np.abs(a - b).max() < 5e-3
+    assert np.abs(a8.astype(int) - b8.astype(int)).max() <= 2
