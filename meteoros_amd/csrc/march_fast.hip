@@ -168,7 +168,23 @@ struct FastViews {
   unsigned int calib_mask;    // bins whose held-out error is too high
   float fallback_density;     // local density above which the cone runs
   int audit;                  // also cone-march one in `audit` answered steps (power of two, 0 = off)
+  int xcd_remap;              // deal contiguous tile bands to each XCD
 };
+
+// Workgroups go to the 8 XCDs round-robin (workgroup w to XCD w % 8).  With
+// remap on, XCD k takes a contiguous band of the tile grid instead, so
+// neighbouring rays, which read neighbouring texels, share one L2.  The map
+// is a bijection on the grid (the tail beyond a multiple of 8 is kept).
+__device__ __forceinline__ int2 tile_of_block(int remap) {
+  const int nx = (int)gridDim.x, n = (int)(gridDim.x * gridDim.y);
+  const int w = (int)(blockIdx.y * gridDim.x + blockIdx.x);
+  int t = w;
+  if (remap) {
+    const int per = n / 8;
+    if (w < per * 8) t = (w % 8) * per + w / 8;
+  }
+  return make_int2(t % nx, t / nx);
+}
 
 // Shell entry/exit of a ray (raySphereIntersection, :229-273, with the
 // t = length(point_world - rO_normalised) quirk of :269).
@@ -225,6 +241,7 @@ struct RayMarch {
   Ray ray;
   f3 c, inner, wk, woff;
   float start_t, end_t, stepSize, cos_angle, HG_light, inv_thick, inv_div, ls, ws, cone_scale;
+  float cos_ratio;  // (cos_angle - 0.7) / (1 - 0.7) of GetLightEnergy's first remap
   int pixelID, cap;
 };
 
@@ -337,6 +354,7 @@ __device__ __forceinline__ void setup_march(const FrameConsts& fc, const Ray& ra
     rm.stepSize = (rm.end_t - rm.start_t) / maxSteps;
     rm.cos_angle = dot(normalize(ray.d), ld3(fc.light_dir));
     rm.HG_light = fmaxf(hg(rm.cos_angle, fc.hg_g), fc.hg_si * hg(rm.cos_angle, fc.hg_g2));
+    rm.cos_ratio = (rm.cos_angle - 0.7f) / (1.0f - 0.7f);
     rm.inv_thick = 1.0f / fc.thickness;
     rm.inv_div = 1.0f / fc.sample_divisor;
     rm.ls = length(startPos - ray.o);
@@ -422,6 +440,14 @@ __device__ __forceinline__ void march_tile(const FrameConsts& fc, const FastView
   const bool active = px < W && py < H;
   const int nls = fc.light_samples;
   ConeLds& L = cone_lds[wave];
+  // cone work map: each round covers cgroups = 64 / nls steps; lane l does
+  // sample ci = l % nls of slot cs = l / nls, its kernel vector in registers
+  const int cgroups = nls > 0 ? 64 / nls : 1;
+  const int ci = nls > 0 ? lane % nls : 0, cs = nls > 0 ? lane / nls : 64;
+  const bool cone_lane = nls > 0 && cs < cgroups;
+  const float fci = (float)ci;
+  const float kx = cone_lane ? skern[3 * ci] : 0.0f, ky = cone_lane ? skern[3 * ci + 1] : 0.0f,
+              kz = cone_lane ? skern[3 * ci + 2] : 0.0f;
   unsigned long long n_steps = 0, n_cloud = 0, n_cone = 0, n_cone_er = 0, n_early = 0;
   unsigned long long n_ocean = 0, n_sky = 0, n_marched = 0, n_mlp = 0, n_mlp_fb = 0, n_audit = 0;
   float audit_sq = 0.0f;
@@ -528,12 +554,14 @@ __device__ __forceinline__ void march_tile(const FrameConsts& fc, const FastView
       __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
       __builtin_amdgcn_wave_barrier();
       __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
-      const int total = ncl * nls;
-      for (int w = lane; w < total; w += 64) {
-        const int r = w / nls, i = w - r * nls;
+      // round q: lane l takes sample ci of step q * cgroups + cs
+      for (int r0 = 0; r0 < ncl; r0 += cgroups) {
+        const int r = r0 + cs;
+        if (!cone_lane || r >= ncl) continue;
+        const int w = r * nls + ci;
         const float4 a = L.own[r][0], b = L.own[r][1];
-        const float s = (float)i * a.w;
-        const f3 slp = mk(a.x + skern[3 * i] * s, a.y + skern[3 * i + 1] * s, a.z + skern[3 * i + 2] * s);
+        const float s = fci * a.w;
+        const f3 slp = mk(a.x + kx * s, a.y + ky * s, a.z + kz * s);
         float cov = fc.coverage_const, gw = 1.0f;
         if (COV == MTC_COVERAGE_WEATHER) {
           const float2 cw = sample_map<F16>(fv.weather, fv.weather_sh, slp.x * fc.weather_scale,
@@ -569,9 +597,11 @@ __device__ __forceinline__ void march_tile(const FrameConsts& fc, const FastView
         // GetLightEnergy (:331-388): exp(-dl) once for both identical terms
         const float h = pr.h;
         float prim = fast_exp(-dal);
-        float att = fmaxf(remap(rm.cos_angle, 0.7f, 1.0f, prim, prim * 0.25f), prim);
-        float depth = 0.05f + fast_pow(pr.base, clampf(remap(h * 0.125f, 0.3f, 0.85f, 0.5f, 2.0f), 0.5f, 2.0f));
-        float vert = fast_pow(clampf(remap(h * 1.5f, 0.07f, 0.34f, 0.1f, 1.0f), 0.1f, 1.0f), 0.8f);
+        // remap(cos, 0.7, 1, prim, 0.25 prim) with its per-ray ratio hoisted;
+        // the two height remaps multiply by the reciprocal of their constant span
+        float att = fmaxf(prim + rm.cos_ratio * (prim * 0.25f - prim), prim);
+        float depth = 0.05f + fast_pow(pr.base, clampf(0.5f + ((h * 0.125f - 0.3f) * (1.0f / 0.55f)) * 1.5f, 0.5f, 2.0f));
+        float vert = fast_pow(clampf(0.1f + ((h * 1.5f - 0.07f) * (1.0f / 0.27f)) * 0.9f, 0.1f, 1.0f), 0.8f);
         E = att * prim * (depth * vert) * rm.HG_light * fc.brightness;
       }
       // the next chunk rewrites the work area
@@ -590,12 +620,13 @@ __device__ __forceinline__ void march_tile(const FrameConsts& fc, const FastView
       if (COUNT && j == 0) n_steps += (unsigned)__popc(gm);
       gdone = ((gm >> (G - 1)) & 1u) == 0u;
     } else if (!gdone) {
+      const int step_flags = (pr.valid ? 1 : 0) | (pr.cloud ? 2 : 0);
       bool last_valid = false, exited = false;
 #pragma unroll
       for (int i = 0; i < G; ++i) {
         const int src = gbase + i;
-        const bool v = G == 1 ? pr.valid : (bool)__shfl((int)pr.valid, src, 64);
-        const bool cl = G == 1 ? pr.cloud : (bool)__shfl((int)pr.cloud, src, 64);
+        const int fl = G == 1 ? step_flags : __shfl(step_flags, src, 64);
+        const bool v = (fl & 1) != 0, cl = (fl & 2) != 0;
         const float da = G == 1 ? pr.da : __shfl(pr.da, src, 64);
         const float e = G == 1 ? E : __shfl(E, src, 64);
         const int ch = COUNT ? (G == 1 ? cone_hits : __shfl(cone_hits, src, 64)) : 0;
@@ -667,7 +698,8 @@ __device__ __forceinline__ void march_tile(const FrameConsts& fc, const FastView
 
 // One frame: a 256-lane workgroup marches an 8 x (32 / G) tile of rays.
 template <int COV, bool F16, bool FP64SHELL, bool COUNT, int G>
-__global__ __launch_bounds__(256) void march_kernel(FrameConsts fc, FastViews fv, OutputViews ov, int full) {
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(COUNT ? 1 : 5))) void march_kernel(
+    FrameConsts fc, FastViews fv, OutputViews ov, int full) {
   __shared__ float sjit[48];
   __shared__ float skern[18];
   __shared__ ConeLds cone_lds[4];
@@ -675,8 +707,9 @@ __global__ __launch_bounds__(256) void march_kernel(FrameConsts fc, FastViews fv
   if (threadIdx.x < 18) skern[threadIdx.x] = (&fc.kern[0][0])[threadIdx.x];
   __syncthreads();
   constexpr int TX = 8, TY = 256 / G / 8;
-  march_tile<COV, F16, FP64SHELL, COUNT, G>(fc, fv, ov.color, ov.mask, ov.counters, full, blockIdx.x * TX,
-                                            blockIdx.y * TY, TX, sjit, skern, cone_lds);
+  const int2 tile = tile_of_block(fv.xcd_remap);
+  march_tile<COV, F16, FP64SHELL, COUNT, G>(fc, fv, ov.color, ov.mask, ov.counters, full, tile.x * TX, tile.y * TY,
+                                            TX, sjit, skern, cone_lds);
 }
 
 // One frame with learned light (fast variant's sampling): the in-cloud
@@ -691,8 +724,9 @@ __global__ __launch_bounds__(256) void march_learned_kernel(FrameConsts fc, Fast
   if (threadIdx.x < 18) skern[threadIdx.x] = (&fc.kern[0][0])[threadIdx.x];
   __syncthreads();
   constexpr int TX = 8, TY = 256 / G / 8;
-  march_tile<COV, false, true, COUNT, G, true>(fc, fv, ov.color, ov.mask, ov.counters, full, blockIdx.x * TX,
-                                               blockIdx.y * TY, TX, sjit, skern, cone_lds, mlp_lds);
+  const int2 tile = tile_of_block(fv.xcd_remap);
+  march_tile<COV, false, true, COUNT, G, true>(fc, fv, ov.color, ov.mask, ov.counters, full, tile.x * TX,
+                                               tile.y * TY, TX, sjit, skern, cone_lds, mlp_lds);
 }
 
 // Training samples for the learned light: every pixel of the view is
@@ -901,8 +935,9 @@ static bool launch_g(int lanes_per_ray, hipStream_t s, const FrameConsts& fc, co
 // filtering); lanes_per_ray in {1, 2, 4, 8}
 hipError_t launch_march_fast(const FrameConsts& fc, const uint4* lf, const uint4* hf, const uint4* curl,
                              const uint4* weather, const VolumeViews& vv, const OutputViews& ov, int full, int variant,
-                             int lanes_per_ray, hipStream_t stream) {
+                             int lanes_per_ray, int xcd_remap, hipStream_t stream) {
   mtc::fast::FastViews fv{lf, hf, curl, weather, vv.lf_shift, vv.hf_shift, vv.curl_shift, vv.weather_shift};
+  fv.xcd_remap = xcd_remap;
   // Both variants keep the shell quadratic in FP64: in FP32 its hit points
   // move by 0.3-1 m (C = dot(rO, rO) - 1 cancels), which shifts every t_k and
   // flips the int(t) jitter index of most steps (:631).
@@ -941,8 +976,9 @@ static void launch_learned_v(hipStream_t s, const FrameConsts& fc, const mtc::fa
 // Learned-light frame (fast sampling, FP64 shell); lanes_per_ray 4 or 8.
 hipError_t launch_march_learned(const FrameConsts& fc, const uint4* lf, const uint4* hf, const uint4* curl,
                                 const uint4* weather, const VolumeViews& vv, const OutputViews& ov, int full,
-                                int lanes_per_ray, const LearnedLight& ll, hipStream_t s) {
-  const mtc::fast::FastViews fv = learned_views(lf, hf, curl, weather, vv, ll);
+                                int lanes_per_ray, int xcd_remap, const LearnedLight& ll, hipStream_t s) {
+  mtc::fast::FastViews fv = learned_views(lf, hf, curl, weather, vv, ll);
+  fv.xcd_remap = xcd_remap;
   const bool wx = fc.coverage_mode == MTC_COVERAGE_WEATHER;
   if (lanes_per_ray == 8) {
     if (wx) launch_learned_v<MTC_COVERAGE_WEATHER, 8>(s, fc, fv, ov, full);
