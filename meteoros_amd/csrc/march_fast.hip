@@ -168,23 +168,7 @@ struct FastViews {
   unsigned int calib_mask;    // bins whose held-out error is too high
   float fallback_density;     // local density above which the cone runs
   int audit;                  // also cone-march one in `audit` answered steps (power of two, 0 = off)
-  int xcd_remap;              // deal contiguous tile bands to each XCD
 };
-
-// Workgroups go to the 8 XCDs round-robin (workgroup w to XCD w % 8).  With
-// remap on, XCD k takes a contiguous band of the tile grid instead, so
-// neighbouring rays, which read neighbouring texels, share one L2.  The map
-// is a bijection on the grid (the tail beyond a multiple of 8 is kept).
-__device__ __forceinline__ int2 tile_of_block(int remap) {
-  const int nx = (int)gridDim.x, n = (int)(gridDim.x * gridDim.y);
-  const int w = (int)(blockIdx.y * gridDim.x + blockIdx.x);
-  int t = w;
-  if (remap) {
-    const int per = n / 8;
-    if (w < per * 8) t = (w % 8) * per + w / 8;
-  }
-  return make_int2(t % nx, t / nx);
-}
 
 // Shell entry/exit of a ray (raySphereIntersection, :229-273, with the
 // t = length(point_world - rO_normalised) quirk of :269).
@@ -707,9 +691,8 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(COUNT ? 1 :
   if (threadIdx.x < 18) skern[threadIdx.x] = (&fc.kern[0][0])[threadIdx.x];
   __syncthreads();
   constexpr int TX = 8, TY = 256 / G / 8;
-  const int2 tile = tile_of_block(fv.xcd_remap);
-  march_tile<COV, F16, FP64SHELL, COUNT, G>(fc, fv, ov.color, ov.mask, ov.counters, full, tile.x * TX, tile.y * TY,
-                                            TX, sjit, skern, cone_lds);
+  march_tile<COV, F16, FP64SHELL, COUNT, G>(fc, fv, ov.color, ov.mask, ov.counters, full, blockIdx.x * TX,
+                                            blockIdx.y * TY, TX, sjit, skern, cone_lds);
 }
 
 // One frame with learned light (fast variant's sampling): the in-cloud
@@ -724,9 +707,8 @@ __global__ __launch_bounds__(256) void march_learned_kernel(FrameConsts fc, Fast
   if (threadIdx.x < 18) skern[threadIdx.x] = (&fc.kern[0][0])[threadIdx.x];
   __syncthreads();
   constexpr int TX = 8, TY = 256 / G / 8;
-  const int2 tile = tile_of_block(fv.xcd_remap);
-  march_tile<COV, false, true, COUNT, G, true>(fc, fv, ov.color, ov.mask, ov.counters, full, tile.x * TX,
-                                               tile.y * TY, TX, sjit, skern, cone_lds, mlp_lds);
+  march_tile<COV, false, true, COUNT, G, true>(fc, fv, ov.color, ov.mask, ov.counters, full, blockIdx.x * TX,
+                                               blockIdx.y * TY, TX, sjit, skern, cone_lds, mlp_lds);
 }
 
 // Training samples for the learned light: every pixel of the view is
@@ -935,9 +917,8 @@ static bool launch_g(int lanes_per_ray, hipStream_t s, const FrameConsts& fc, co
 // filtering); lanes_per_ray in {1, 2, 4, 8}
 hipError_t launch_march_fast(const FrameConsts& fc, const uint4* lf, const uint4* hf, const uint4* curl,
                              const uint4* weather, const VolumeViews& vv, const OutputViews& ov, int full, int variant,
-                             int lanes_per_ray, int xcd_remap, hipStream_t stream) {
+                             int lanes_per_ray, hipStream_t stream) {
   mtc::fast::FastViews fv{lf, hf, curl, weather, vv.lf_shift, vv.hf_shift, vv.curl_shift, vv.weather_shift};
-  fv.xcd_remap = xcd_remap;
   // Both variants keep the shell quadratic in FP64: in FP32 its hit points
   // move by 0.3-1 m (C = dot(rO, rO) - 1 cancels), which shifts every t_k and
   // flips the int(t) jitter index of most steps (:631).
@@ -976,9 +957,8 @@ static void launch_learned_v(hipStream_t s, const FrameConsts& fc, const mtc::fa
 // Learned-light frame (fast sampling, FP64 shell); lanes_per_ray 4 or 8.
 hipError_t launch_march_learned(const FrameConsts& fc, const uint4* lf, const uint4* hf, const uint4* curl,
                                 const uint4* weather, const VolumeViews& vv, const OutputViews& ov, int full,
-                                int lanes_per_ray, int xcd_remap, const LearnedLight& ll, hipStream_t s) {
-  mtc::fast::FastViews fv = learned_views(lf, hf, curl, weather, vv, ll);
-  fv.xcd_remap = xcd_remap;
+                                int lanes_per_ray, const LearnedLight& ll, hipStream_t s) {
+  const mtc::fast::FastViews fv = learned_views(lf, hf, curl, weather, vv, ll);
   const bool wx = fc.coverage_mode == MTC_COVERAGE_WEATHER;
   if (lanes_per_ray == 8) {
     if (wx) launch_learned_v<MTC_COVERAGE_WEATHER, 8>(s, fc, fv, ov, full);

@@ -15,7 +15,6 @@
 #include <hip/hip_runtime.h>
 
 #include <cstdio>
-#include <cstdlib>
 #include <cstring>
 #include <array>
 #include <new>
@@ -29,10 +28,10 @@ hipError_t launch_march_plain(const FrameConsts& fc, const VolumeViews& vv, cons
                               hipStream_t stream);
 hipError_t launch_march_fast(const FrameConsts& fc, const uint4* lf, const uint4* hf, const uint4* curl,
                              const uint4* weather, const VolumeViews& vv, const OutputViews& ov, int full, int variant,
-                             int lanes_per_ray, int xcd_remap, hipStream_t stream);
+                             int lanes_per_ray, hipStream_t stream);
 hipError_t launch_march_learned(const FrameConsts& fc, const uint4* lf, const uint4* hf, const uint4* curl,
                                 const uint4* weather, const VolumeViews& vv, const OutputViews& ov, int full,
-                                int lanes_per_ray, int xcd_remap, const LearnedLight& ll, hipStream_t s);
+                                int lanes_per_ray, const LearnedLight& ll, hipStream_t s);
 hipError_t pack_fast_resources(const VolumeViews& vv, uint4* lf, uint4* hf, uint4* curl, uint4* weather,
                                hipStream_t s);
 hipError_t launch_reproject(const FrameConsts& fc, const OutputViews& ov, int mode, hipStream_t stream);
@@ -145,19 +144,16 @@ int launch_march(mtc_ctx* c, const FrameConsts& fc, const OutputViews& ov, int f
   switch (c->desc.kernel) {
     case MTC_KERNEL_PLAIN: e = launch_march_plain(fc, vv, ov, full, s); break;
     case MTC_KERNEL_FAST:
-      e = launch_march_fast(fc, c->q_lf, c->q_hf, c->q_curl, c->q_weather, vv, ov, full, 0, c->lanes_per_ray,
-                            c->xcd_remap, s);
+      e = launch_march_fast(fc, c->q_lf, c->q_hf, c->q_curl, c->q_weather, vv, ov, full, 0, c->lanes_per_ray, s);
       break;
     case MTC_KERNEL_PRECISION:
-      e = launch_march_fast(fc, c->q_lf, c->q_hf, c->q_curl, c->q_weather, vv, ov, full, 1, c->lanes_per_ray,
-                            c->xcd_remap, s);
+      e = launch_march_fast(fc, c->q_lf, c->q_hf, c->q_curl, c->q_weather, vv, ov, full, 1, c->lanes_per_ray, s);
       break;
     case MTC_KERNEL_LEARNED: {
       LearnedLight ll;
       int rc = learned_light_of(c, &ll);
       if (rc) return rc;
-      e = launch_march_learned(fc, c->q_lf, c->q_hf, c->q_curl, c->q_weather, vv, ov, full, c->lanes_per_ray,
-                               c->xcd_remap, ll, s);
+      e = launch_march_learned(fc, c->q_lf, c->q_hf, c->q_curl, c->q_weather, vv, ov, full, c->lanes_per_ray, ll, s);
       break;
     }
     default: return mtc_fail(MTC_ERR_UNSUPPORTED, "march: kernel variant not available in this build");
@@ -230,12 +226,6 @@ int mtc_create(const mtc_desc* d, mtc_ctx** out) {
   if (!c) return mtc_fail(MTC_ERR_OUT_OF_MEMORY, "mtc_create: host allocation failed");
   c->desc = *d;
   c->lanes_per_ray = d->lanes_per_ray ? d->lanes_per_ray : 8;
-  {
-    // METEOROS_XCD_REMAP=0 deals tiles to XCDs round-robin (the hardware
-    // order) instead of in contiguous bands
-    const char* v = std::getenv("METEOROS_XCD_REMAP");
-    c->xcd_remap = (v && v[0] == '0') ? 0 : 1;
-  }
   c->lf_bytes = (size_t)v.lf_dim * v.lf_dim * v.lf_dim * texel_bytes(v.format);
   c->hf_bytes = (size_t)v.hf_dim * v.hf_dim * v.hf_dim * texel_bytes(v.format);
   c->curl_bytes = (size_t)v.curl_dim * v.curl_dim * 4;

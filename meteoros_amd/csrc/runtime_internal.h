@@ -149,7 +149,6 @@ struct mtc_ctx {
   uint4* q_weather = nullptr;
   int latest = 0;
   int lanes_per_ray = 8;  // fast kernels: lanes cooperating on one ray
-  int xcd_remap = 1;      // fast kernels: contiguous tile bands per XCD
   bool volumes_ready = false;
   // MTC_FLAG_TIMING: event triples (before reprojection, between, after
   // march) of frames not yet folded into the totals
