@@ -50,6 +50,20 @@ FLOPS = {
 }
 
 
+def hbm_traffic(kernel_name: str):
+    """HBM bytes per march launch of `kernel_name` from the committed PMC
+    summary (profiles/march_pmc_1080p.json: FETCH_SIZE and WRITE_SIZE of
+    rocprofv3 --pmc runs, each scaled by what the counter reads for a copy
+    kernel of known size), or None when the summary has no such entry."""
+    path = os.path.join(ROOT, "profiles", "march_pmc_1080p.json")
+    try:
+        with open(path) as f:
+            k = json.load(f)["kernels"][kernel_name]
+        return float(k["fetch_bytes"] + k["write_bytes"])
+    except (OSError, KeyError, ValueError):
+        return None
+
+
 def march_flops(c: dict) -> float:
     return (c["rays_marched"] * FLOPS["ray_marched"] + c["rays_ocean"] * FLOPS["ray_ocean"]
             + c["rays_sky"] * FLOPS["ray_sky"] + c["primary_steps"] * FLOPS["primary_step"]
@@ -84,32 +98,32 @@ _NET = None
 PSNR_BAR_DB = 45.0
 
 
-def light_net(seed_poses: int = 3):
+def light_net(steps: int = 4000, seed_poses: int = 3):
     """The learned light's network, trained once per process on the GPU:
-    samples from 16 seeded poses of the workload's scene at 480x270 (yaw
-    spread over the circle, pitch 8-40 deg; 2^22 samples), 4000 Adam steps
-    of 8192, held-out calibration (bins with RMS error above 0.25 go to the
-    cone).  Untimed set-up."""
+    cone-march samples from 16 seeded poses of the workload's scene at
+    960x540 (yaw spread over the circle, pitch 8-40 deg; 2^24 samples kept),
+    `steps` Adam steps of 8192, held-out calibration (bins with RMS error
+    above 0.25 go to the cone).  Untimed set-up."""
     global _NET
     if _NET is None:
         t0 = time.perf_counter()
         params, sun = M.preset_params(PRESET)
-        net = M.LightNet(capacity=1 << 22)
+        net = M.LightNet(capacity=1 << 24)
         rng = np.random.default_rng(seed_poses)
-        with M.CloudStage(480, 270, M.VolumeSpec(128, 32, 128, 512, A.MTC_FMT_UNORM8), kernel=A.MTC_KERNEL_FAST) as st:
+        with M.CloudStage(960, 540, M.VolumeSpec(128, 32, 128, 512, A.MTC_FMT_UNORM8), kernel=A.MTC_KERNEL_FAST) as st:
             st.generate_volumes(1, 2)
             for i in range(16):
                 p = np.radians(rng.uniform(8.0, 40.0))
                 y = np.radians(22.5 * i)
                 cam = M.Camera((0.0, 0.0, 0.0), (float(np.cos(p) * np.sin(y)), float(np.sin(p)),
-                                                 float(-np.cos(p) * np.cos(y))), 480, 270, 45.0)
+                                                 float(-np.cos(p) * np.cos(y))), 960, 540, 45.0)
                 t = M.Time()
                 t.ubo.frame_count_mod16 = int(rng.integers(16))
                 t.ubo.time[1] = float(rng.uniform(0.0, 2.0))
                 net.collect(st, cam.ubo(), t.ubo, sun, params, seed=100 + i, keep_fraction=0.3)
         n = net.sample_count()
         hold = min(65536, n // 8)
-        curve = net.train(steps=4000, batch=8192, lr=2e-3, seed=3, holdout=hold)
+        curve = net.train(steps=steps, batch=8192, lr=2e-3, seed=3, holdout=hold)
         cal = net.calibrate(holdout=hold, max_rms=0.25, fallback_density=0.8)
         _NET = (net, {"samples": n, "train_loss_last100": float(curve[-100:].mean()),
                       "heldout_rms": round(float(cal["rms_all"]), 4),
@@ -119,13 +133,16 @@ def light_net(seed_poses: int = 3):
     return _NET
 
 
+LIGHT_STEPS = 4000
+
+
 def run_gpu(kernel: int, seq, warmup: int, steps: int, flags: int = 0, lanes: int = 0):
     params, sun = M.preset_params(PRESET)
     st = M.CloudStage(W, H, M.VolumeSpec(128, 32, 128, 512, A.MTC_FMT_UNORM8), kernel=kernel, flags=flags,
                       lanes_per_ray=lanes)
     st.generate_volumes(1, 2)
     if kernel == A.MTC_KERNEL_LEARNED:
-        light_net()[0].attach(st)
+        light_net(LIGHT_STEPS)[0].attach(st)
     for cur, old, tm in seq[:warmup]:
         st.frame(cur, old, tm, sun, params)
     st.synchronize()
@@ -218,6 +235,8 @@ def main() -> int:
     ap.add_argument("--kernel", default="auto", choices=["auto"] + sorted(KERNELS),
                     help="auto: time every kernel (the learned one after training its network), report the "
                          "fastest with PSNR >= 45 dB vs the FP64 oracle")
+    ap.add_argument("--light-steps", type=int, default=4000,
+                    help="Adam steps of the learned light's network (untimed set-up; fewer for profiler runs)")
     ap.add_argument("--cpu-seconds", type=float, default=12.0)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     args = ap.parse_args()
@@ -228,6 +247,8 @@ def main() -> int:
     if args.steps < 1 or args.warmup < 0:
         print(json.dumps({"error": "--steps must be >= 1 and --warmup >= 0"}))
         return 2
+    global LIGHT_STEPS
+    LIGHT_STEPS = max(1, args.light_steps)
     seq = make_sequence(args.warmup + args.steps)
     work = count_work(seq, args.warmup, args.steps)
     names = sorted(KERNELS, key=lambda n: KERNELS[n]) if args.kernel == "auto" else [args.kernel]
@@ -267,7 +288,7 @@ def main() -> int:
     flops_per_launch = march_flops(work) / args.steps
     achieved = flops_per_launch / (march_ms * 1e-3) / 1e12
     roof = {"bound": "mfma", "pipe": "valu-fp32", "kernel": "march", "achieved": round(achieved, 3),
-            "peak": 157.3, "unit": "TFLOP/s", "frac": round(achieved / 157.3, 4), "traffic": None,
+            "peak": 157.3, "unit": "TFLOP/s", "frac": round(achieved / 157.3, 4), "traffic": hbm_traffic(best),
             "flops_per_launch": flops_per_launch, "avg_launch_ms": round(march_ms, 4)}
     reproj_bytes = W * H * 16 * 2
     per_kernel = {}

@@ -128,6 +128,63 @@ __device__ __forceinline__ float fast_pow(float x, float y) { return __builtin_a
 __device__ __forceinline__ float fast_exp(float x) { return __builtin_amdgcn_exp2f(x * 1.4426950408889634f); }
 __device__ __forceinline__ float rcp(float x) { return __builtin_amdgcn_rcpf(x); }
 
+// castRay (:194-226) with the camera basis and eye + look taken from the
+// per-frame constants (the host computes them with the same FP32
+// operations, so the ray is bit-identical to the per-ray form).
+__device__ __forceinline__ Ray cast_ray_folded(const FrameConsts& fc, float ux, float uy, float jx, float jy) {
+  float nx = ux * 2.0f - 1.0f, ny = uy * 2.0f - 1.0f;
+  nx = nx + jx;
+  ny = ny + jy;
+  const f3 cx = (nx * fc.tfx) * ld3(fc.rp_right);
+  const f3 cy = (ny * fc.tfy) * ld3(fc.rp_up);
+  const f3 eye = ld3(fc.eye);
+  const f3 p = (ld3(fc.rp_ref) + cx) + cy;
+  Ray r;
+  r.o = eye;
+  r.d = normalize(p - eye);
+  return r;
+}
+
+// HenyeyGreenstein (:295-300) with b^1.5 as b sqrt(b) and a reciprocal.
+__device__ __forceinline__ float hg_fast(float c, float g) {
+  const float b = 1.0f + g * g - 2.0f * g * c;
+  return (1.0f - g * g) * rcp(b * __builtin_amdgcn_sqrtf(b)) * 0.07957747154594767f;
+}
+
+// getAtmosphereColorPhysical (:427-467) on the transcendental unit:
+// cos(acos(y)) = y, pow(x, 1.5) = x sqrt(x), pow(x, 0.5) = sqrt(x), the
+// zenith pow and the three exps as exp2/log2, divisions by per-frame
+// constants as reciprocals; the shader's operations otherwise.  The IEEE
+// library pow alone costs ~165 instructions and the sky runs once per ray,
+// i.e. once per wave for all of its rays.
+__device__ __forceinline__ f3 atmosphere_fast(const FrameConsts& fc, f3 dir) {
+  const f3 BR = ld3(fc.beta_r), BM = ld3(fc.beta_m), BS = ld3(fc.beta_sum);
+  const f3 sunDir = ld3(fc.sky_sun_dir);
+  const float sunE = fc.sunE;
+  const float y = fmaxf(0.0f, dir.y);
+  const float zenith = acosf(y);
+  const float inv = rcp(y + 0.15f * fast_pow(93.885f - zenith * (180.0f / 3.14159265f), -1.253f));
+  const float sR = 8.4E3f * inv, sM = 1.25E3f * inv;
+  const float cosTheta = dot(sunDir, dir);
+  const float cc = cosTheta * 0.5f + 0.5f;
+  const float rPhase = 0.05968310365946075f * (1.0f + cc * cc);
+  const float mPhase = hg_fast(cosTheta, 0.8f);
+  const float yc = fc.ydot_clamped;
+  const float disk = sunE * 15000.0f * smoothstepf(fc.sun_disk_lo, fc.sun_disk_hi, cosTheta);
+  const float br[3] = {BR.x, BR.y, BR.z}, bm[3] = {BM.x, BM.y, BM.z}, bs[3] = {BS.x, BS.y, BS.z};
+  const float add[3] = {0.0f, 0.0003f, 0.00075f};
+  float o[3];
+#pragma unroll
+  for (int c = 0; c < 3; ++c) {
+    const float fex = fast_exp(-br[c] * sR + bm[c] * sM);
+    const float sb = sunE * ((br[c] * rPhase + bm[c] * mPhase) * rcp(bs[c]));
+    const float a = sb * (1.0f - fex), b = sb * fex;
+    const float lin = a * __builtin_amdgcn_sqrtf(a) * mixf(1.0f, __builtin_amdgcn_sqrtf(b), yc);
+    o[c] = (lin + (0.1f * fex + disk * fex)) * 0.04f + add[c];
+  }
+  return mk(o[0], o[1], o[2]);
+}
+
 // Weather mode: the height-gradient weight of a sample, mix(a, b, type)
 // * 0.5 (getDensityHeightGradientForPoint :475-487 and the x0.5 of the
 // weather path), from the step's two profiles sc, st.
@@ -183,19 +240,22 @@ __device__ __forceinline__ void shell_span(const FrameConsts& fc, const Ray& ray
     double cx = c.x, cy = -fc.earth_r_d, cz = c.z;
     double ts[2] = {0.0, 0.0};
     double sp[3] = {0.0, 0.0, 0.0};
+    // divisions as products with one reciprocal each (1 ulp in FP64)
+    const double A = dx * dx + dy * dy + dz * dz;
+    const double inv2A = 1.0 / (2.0 * A);
 #pragma unroll
     for (int k = 0; k < 2; ++k) {
-      double R = k == 0 ? fc.r_in_d : fc.r_out_d;
-      double rx0 = (ox - cx) / R, ry0 = (oy - cy) / R, rz0 = (oz - cz) / R;
-      double A = dx * dx + dy * dy + dz * dz;
+      const double R = k == 0 ? fc.r_in_d : fc.r_out_d;
+      const double iR = 1.0 / R;
+      double rx0 = (ox - cx) * iR, ry0 = (oy - cy) * iR, rz0 = (oz - cz) * iR;
       double B = 2.0 * (dx * rx0 + dy * ry0 + dz * rz0);
       double C = (rx0 * rx0 + ry0 * ry0 + rz0 * rz0) - 1.0;
       double disc = B * B - 4.0 * A * C;
       double t = -1.0;
       if (disc >= 0.0) {
         double sq = sqrt(disc);
-        t = (-B - sq) / (2.0 * A);
-        if (t < 0.0) t = (-B + sq) / (2.0 * A);
+        t = (-B - sq) * inv2A;
+        if (t < 0.0) t = (-B + sq) * inv2A;
       }
       if (t >= 0.0) {
         double pxw = (rx0 + t * dx) * R + cx, pyw = (ry0 + t * dy) * R + cy, pzw = (rz0 + t * dz) * R + cz;
@@ -264,7 +324,7 @@ __device__ __forceinline__ Prim eval_primary(const FrameConsts& fc, const FastVi
   const f3 sp = rel * rm.inv_div;
   const f3 pe = pos - ray.o;
   const f3 pc = pos - rm.c;
-  const float lr = __builtin_sqrtf(dot(pe, pe));
+  const float lr = __builtin_amdgcn_sqrtf(dot(pe, pe));  // v_sqrt_f32 (1 ulp); the plain kernel keeps IEEE
   const float cosT = dot(ray.d, pc) * __builtin_amdgcn_rsqf(dot(pc, pc));
   const float h = fabsf(cosT * (lr - rm.ls)) * rm.inv_thick;
   const f3 sk = sp + h * rm.wk + rm.woff;
@@ -336,8 +396,8 @@ __device__ __forceinline__ void setup_march(const FrameConsts& fc, const Ray& ra
                          ? (float)fc.steps_max
                          : floorf(mixf((float)fc.steps_min, (float)fc.steps_max, 1.0f - d));
     rm.stepSize = (rm.end_t - rm.start_t) / maxSteps;
-    rm.cos_angle = dot(normalize(ray.d), ld3(fc.light_dir));
-    rm.HG_light = fmaxf(hg(rm.cos_angle, fc.hg_g), fc.hg_si * hg(rm.cos_angle, fc.hg_g2));
+    rm.cos_angle = dot(ray.d * __builtin_amdgcn_rsqf(dot(ray.d, ray.d)), ld3(fc.light_dir));
+    rm.HG_light = fmaxf(hg_fast(rm.cos_angle, fc.hg_g), fc.hg_si * hg_fast(rm.cos_angle, fc.hg_g2));
     rm.cos_ratio = (rm.cos_angle - 0.7f) / (1.0f - 0.7f);
     rm.inv_thick = 1.0f / fc.thickness;
     rm.inv_div = 1.0f / fc.sample_divisor;
@@ -446,7 +506,7 @@ __device__ __forceinline__ void march_tile(const FrameConsts& fc, const FastView
   ray.d = mk(0.0f, 1.0f, 0.0f);
   if (active) {
     float ux = (float)px / (float)W, uy = 1.0f - (float)py / (float)H;
-    ray = cast_ray(fc, fc.view, ux, uy, fc.jit_march[0], fc.jit_march[1]);
+    ray = cast_ray_folded(fc, ux, uy, fc.jit_march[0], fc.jit_march[1]);
     const float d = ray.d.y;
     if (d < 0.0f) {
       float a = -ray.d.y * 5.5f;
@@ -454,7 +514,7 @@ __device__ __forceinline__ void march_tile(const FrameConsts& fc, const FastView
                           mixf(0.51f * 0.4f, 0.95f * 0.5f, a), 1.0f);
       n_ocean = 1;
     } else {
-      bg = atmosphere(fc, ray.d) * fmaxf(fc.bg_min, d);
+      bg = atmosphere_fast(fc, ray.d) * fmaxf(fc.bg_min, d);
       if (d < fc.fade_start) {
         color = make_float4(bg.x, bg.y, bg.z, 1.0f);
         n_sky = 1;
@@ -732,7 +792,7 @@ __global__ __launch_bounds__(256) void light_samples_kernel(FrameConsts fc, Fast
   const int py = blockIdx.y * 16 + (threadIdx.x >> 4);
   if (px >= fc.width || py >= fc.height) return;
   const float ux = (float)px / (float)fc.width, uy = 1.0f - (float)py / (float)fc.height;
-  const Ray ray = cast_ray(fc, fc.view, ux, uy, fc.jit_march[0], fc.jit_march[1]);
+  const Ray ray = cast_ray_folded(fc, ux, uy, fc.jit_march[0], fc.jit_march[1]);
   if (ray.d.y < 0.0f || ray.d.y < fc.fade_start) return;
   RayMarch rm;
   setup_march<true>(fc, ray, fc.frame, rm);
