@@ -191,7 +191,9 @@ enum {
 /* Noise and map resources.  Layout of every array: x fastest, then y,
  * then z, tightly packed, 4 channels per texel
  * (ImageLoadingUtility.cpp:75-139, Image.cpp:181-212).  Dimensions must
- * be powers of two (REPEAT wrap is a mask). */
+ * be powers of two (REPEAT wrap is a mask): volumes up to 1024^3 with
+ * MTC_KERNEL_PLAIN, up to 512^3 with the other kernels; maps up to
+ * 8192^2. */
 typedef struct mtc_volume_desc {
   int32_t lf_dim;       /* low-frequency Perlin-Worley volume, lf_dim^3   */
   int32_t hf_dim;       /* high-frequency Worley volume, hf_dim^3         */

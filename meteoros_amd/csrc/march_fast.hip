@@ -27,6 +27,11 @@
 //  * Variant 1 (precision): filtering in packed FP16 (v_pk_fma_f16), FP16
 //    volumes, the shell quadratic in FP64; transmittance and radiance stay
 //    FP32.
+// Multiply-adds contract to fma in this file (the library builds with
+// -ffp-contract=off so that the plain kernel keeps the shader's one
+// rounding per operation); the optimised kernels are held to the FP64
+// oracle, which fma approaches more closely, not to bit equality.
+#pragma clang fp contract(fast)
 #include "march_common.h"
 #include "mlp_common.h"
 
@@ -49,25 +54,37 @@ __device__ __forceinline__ float2 lerp2(float2 p, float2 q, float w) {
 }
 __device__ __forceinline__ h2v lerp2h(h2v p, h2v q, h2v w) { return __builtin_elementwise_fma(w, q - p, p); }
 
+// Footprint entry i of an array as a 32-bit unsigned byte offset from the
+// array base, so the load takes the base from scalar registers and needs
+// no per-lane 64-bit address arithmetic.  Footprint arrays stay below
+// 4 GiB: volumes up to 512^3 (mtc_create rejects larger ones for these
+// kernels), maps up to 8192^2.
+__device__ __forceinline__ uint4 entry(const uint4* __restrict__ q, uint32_t i) {
+  return *(const uint4*)((const char*)q + (i << 4));
+}
+
 struct Coord {
-  int i0;        // linear index of the (x0, y0, z0) footprint entry
-  int dz;        // index offset of the z1 slice (wraps)
+  uint32_t i0;   // linear index of the (x0, y0, z0) footprint entry
+  uint32_t i1;   // linear index of the (x0, y0, z1) entry (z wraps)
   float ax, ay, az;
 };
 
+// s * N - 0.5 as one fma: s * N is exact (N a power of two), so this is
+// the same single rounding as the multiply-then-subtract form.
 __device__ __forceinline__ Coord coord3(int sh, f3 s) {
   const float N = (float)(1 << sh);
-  const int m = (1 << sh) - 1;
-  float ux = s.x * N - 0.5f, uy = s.y * N - 0.5f, uz = s.z * N - 0.5f;
+  const uint32_t m = (1u << sh) - 1u;
+  float ux = __fmaf_rn(s.x, N, -0.5f), uy = __fmaf_rn(s.y, N, -0.5f), uz = __fmaf_rn(s.z, N, -0.5f);
   float fx = floorf(ux), fy = floorf(uy), fz = floorf(uz);
   Coord c;
   c.ax = ux - fx;
   c.ay = uy - fy;
   c.az = uz - fz;
-  int x0 = (int)fx & m, y0 = (int)fy & m, z0 = (int)fz & m;
-  int z1 = (z0 + 1) & m;
-  c.i0 = (((z0 << sh) + y0) << sh) + x0;
-  c.dz = (z1 - z0) << (2 * sh);
+  const uint32_t x0 = (uint32_t)(int)fx & m, y0 = (uint32_t)(int)fy & m, z0 = (uint32_t)(int)fz & m;
+  const uint32_t z1 = (z0 + 1u) & m;
+  const uint32_t xy = (y0 << sh) + x0;
+  c.i0 = (z0 << (2 * sh)) + xy;
+  c.i1 = (z1 << (2 * sh)) + xy;
   return c;
 }
 
@@ -75,8 +92,8 @@ __device__ __forceinline__ Coord coord3(int sh, f3 s) {
 template <bool F16>
 __device__ __forceinline__ float2 sample_lf(const uint4* __restrict__ q, int sh, f3 s) {
   Coord c = coord3(sh, s);
-  uint4 a = q[c.i0];
-  uint4 b = q[c.i0 + c.dz];
+  uint4 a = entry(q, c.i0);
+  uint4 b = entry(q, c.i1);
   if (F16) {
     h2v wx = {(_Float16)c.ax, (_Float16)c.ax}, wy = {(_Float16)c.ay, (_Float16)c.ay},
         wz = {(_Float16)c.az, (_Float16)c.az};
@@ -93,13 +110,9 @@ __device__ __forceinline__ float2 sample_lf(const uint4* __restrict__ q, int sh,
 // HF: single channel, the full 2x2x2 footprint in one entry
 template <bool F16>
 __device__ __forceinline__ float sample_hf(const uint4* __restrict__ q, int sh, f3 s) {
-  const float N = (float)(1 << sh);
-  const int m = (1 << sh) - 1;
-  float ux = s.x * N - 0.5f, uy = s.y * N - 0.5f, uz = s.z * N - 0.5f;
-  float fx = floorf(ux), fy = floorf(uy), fz = floorf(uz);
-  float ax = ux - fx, ay = uy - fy, az = uz - fz;
-  int i = (((((int)fz & m) << sh) + ((int)fy & m)) << sh) + ((int)fx & m);
-  uint4 e = q[i];  // (x0y0z0,x1y0z0) (x0y1z0,x1y1z0) (x0y0z1,x1y0z1) (x0y1z1,x1y1z1)
+  const Coord c = coord3(sh, s);
+  const float ax = c.ax, ay = c.ay, az = c.az;
+  uint4 e = entry(q, c.i0);  // (x0y0z0,x1y0z0) (x0y1z0,x1y1z0) (x0y0z1,x1y0z1) (x0y1z1,x1y1z1)
   float2 p0 = f2(e.x), p1 = f2(e.y), p2 = f2(e.z), p3 = f2(e.w);
   float c0 = __fmaf_rn(ax, p0.y - p0.x, p0.x), c1 = __fmaf_rn(ax, p1.y - p1.x, p1.x);
   float c2 = __fmaf_rn(ax, p2.y - p2.x, p2.x), c3 = __fmaf_rn(ax, p3.y - p3.x, p3.x);
@@ -107,15 +120,16 @@ __device__ __forceinline__ float sample_hf(const uint4* __restrict__ q, int sh, 
   return __fmaf_rn(az, d1 - d0, d0);
 }
 
-// 2D maps: two channels, the 2x2 footprint in one entry
+// 2D maps: two channels, the 2x2 footprint in one entry, sampled at
+// (sx, sy) * scale (maps are at most 8192^2 entries, 1 GiB)
 template <bool F16>
-__device__ __forceinline__ float2 sample_map(const uint4* __restrict__ q, int sh, float sx, float sy) {
-  const float N = (float)(1 << sh);
-  const int m = (1 << sh) - 1;
-  float ux = sx * N - 0.5f, uy = sy * N - 0.5f;
+__device__ __forceinline__ float2 sample_map(const uint4* __restrict__ q, int sh, float sx, float sy, float scale) {
+  const float sN = scale * (float)(1 << sh);
+  const uint32_t m = (1u << sh) - 1u;
+  float ux = __fmaf_rn(sx, sN, -0.5f), uy = __fmaf_rn(sy, sN, -0.5f);
   float fx = floorf(ux), fy = floorf(uy);
   float ax = ux - fx, ay = uy - fy;
-  uint4 e = q[((((int)fy & m) << sh) + ((int)fx & m))];
+  uint4 e = entry(q, ((((uint32_t)(int)fy & m) << sh) + ((uint32_t)(int)fx & m)));
   if (F16) {
     h2v wx = {(_Float16)ax, (_Float16)ax}, wy = {(_Float16)ay, (_Float16)ay};
     h2v r = lerp2h(lerp2h(as_h2(e.x), as_h2(e.y), wx), lerp2h(as_h2(e.z), as_h2(e.w), wx), wy);
@@ -188,10 +202,17 @@ __device__ __forceinline__ f3 atmosphere_fast(const FrameConsts& fc, f3 dir) {
 // Weather mode: the height-gradient weight of a sample, mix(a, b, type)
 // * 0.5 (getDensityHeightGradientForPoint :475-487 and the x0.5 of the
 // weather path), from the step's two profiles sc, st.
+//
+// For type t in [0, 1] the shader's three mixes reduce to one quadratic
+// per half, with D = sc - st:
+//   t <= 0.5:  0.5 st + D t (1.5 - t)
+//   t >  0.5:  0.5 sc + D t (0.5 - t)
+// (both 0.5 sc at t = 0.5; expanding mix(mix(st, sc, 2t), sc, t) and
+// mix(sc, mix(sc, st, 2t - 1), t)).  Seven operations instead of fifteen.
 __device__ __forceinline__ float gradient_weight(float type, float sc, float st) {
-  float a = mixf(st, sc, clampf(type * 2.0f, 0.0f, 1.0f));
-  float b = mixf(sc, st, clampf((type - 0.5f) * 2.0f, 0.0f, 1.0f));
-  return mixf(a, b, type) * 0.5f;
+  const bool lo = type <= 0.5f;
+  const float c = (lo ? st : sc) * 0.5f, k = lo ? 1.5f : 0.5f;
+  return __fmaf_rn((sc - st) * type, k - type, c);
 }
 
 // Density of one LF sample (sampleLowFrequency, :499-540) with coverage
@@ -345,7 +366,7 @@ __device__ __forceinline__ Prim eval_primary(const FrameConsts& fc, const FastVi
   float cov = fc.coverage_const, gw = 1.0f;
   if (COV == MTC_COVERAGE_WEATHER) {
     if (sc == 0.0f && st == 0.0f) return o;
-    const float2 w = sample_map<F16>(fv.weather, fv.weather_sh, sp.x * rm.ws, sp.z * rm.ws);
+    const float2 w = sample_map<F16>(fv.weather, fv.weather_sh, sp.x, sp.z, rm.ws);
     cov = w.x;
     gw = gradient_weight(w.y, sc, st);
     if (gw <= cov) return o;
@@ -356,7 +377,7 @@ __device__ __forceinline__ Prim eval_primary(const FrameConsts& fc, const FastVi
     o.cloud = true;
     // erodeCloudWithHighFrequency at (sk, h): one curl + HF fetch serves the
     // primary erosion and all cone erosions of the step (:645, :665)
-    float2 cn = sample_map<F16>(fv.curl, fv.curl_sh, sk.x, sk.y);
+    float2 cn = sample_map<F16>(fv.curl, fv.curl_sh, sk.x, sk.y, 1.0f);
     float kk = (1.0f - h) * 0.5f;
     f3 p2 = mk(sk.x + cn.x * kk, sk.y + cn.y * kk, sk.z);
     float fh = sample_hf<F16>(fv.hf, fv.hf_sh, p2);
@@ -422,7 +443,7 @@ __device__ __forceinline__ float cone_term(const FrameConsts& fc, const FastView
   const f3 slp = mk(pr.rel.x + skern[3 * i] * s, pr.rel.y + skern[3 * i + 1] * s, pr.rel.z + skern[3 * i + 2] * s);
   float cov = fc.coverage_const, gw = 1.0f;
   if (COV == MTC_COVERAGE_WEATHER) {
-    const float2 cw = sample_map<false>(fv.weather, fv.weather_sh, slp.x * fc.weather_scale, slp.z * fc.weather_scale);
+    const float2 cw = sample_map<false>(fv.weather, fv.weather_sh, slp.x, slp.z, fc.weather_scale);
     cov = cw.x;
     gw = gradient_weight(cw.y, pr.sc, pr.st);
   }
@@ -608,8 +629,7 @@ __device__ __forceinline__ void march_tile(const FrameConsts& fc, const FastView
         const f3 slp = mk(a.x + kx * s, a.y + ky * s, a.z + kz * s);
         float cov = fc.coverage_const, gw = 1.0f;
         if (COV == MTC_COVERAGE_WEATHER) {
-          const float2 cw = sample_map<F16>(fv.weather, fv.weather_sh, slp.x * fc.weather_scale,
-                                            slp.z * fc.weather_scale);
+          const float2 cw = sample_map<F16>(fv.weather, fv.weather_sh, slp.x, slp.z, fc.weather_scale);
           cov = cw.x;
           gw = gradient_weight(cw.y, b.x, b.y);
         }

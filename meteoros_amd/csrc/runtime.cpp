@@ -210,6 +210,10 @@ int mtc_create(const mtc_desc* d, mtc_ctx** out) {
     return mtc_fail(MTC_ERR_INVALID_ARG, "mtc_create: unknown volume format");
   if (d->kernel < MTC_KERNEL_PLAIN || d->kernel > MTC_KERNEL_LEARNED)
     return mtc_fail(MTC_ERR_INVALID_ARG, "mtc_create: unknown kernel variant");
+  if (d->kernel != MTC_KERNEL_PLAIN && (v.lf_dim > 512 || v.hf_dim > 512))
+    return mtc_fail(MTC_ERR_INVALID_ARG,
+                    "mtc_create: the fast, precision and learned kernels take volumes up to 512^3 "
+                    "(32-bit offsets into their footprint copies); MTC_KERNEL_PLAIN takes 1024^3");
   if (d->kernel == MTC_KERNEL_LEARNED && d->lanes_per_ray != 0 && d->lanes_per_ray != 4 && d->lanes_per_ray != 8)
     return mtc_fail(MTC_ERR_INVALID_ARG, "mtc_create: the learned-light kernel takes lanes_per_ray 0, 4 or 8");
   if (d->lanes_per_ray != 0 && d->lanes_per_ray != 1 && d->lanes_per_ray != 2 && d->lanes_per_ray != 4 &&

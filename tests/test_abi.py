@@ -57,6 +57,8 @@ def _desc(**kw):
 @pytest.mark.parametrize("kw", [
     dict(width=0), dict(height=-3), dict(width=20000),
     dict(vol=(30, 16, 32, 64, 0)), dict(vol=(32, 16, 32, 64, 7)), dict(kernel=17), dict(flags=64),
+    dict(vol=(2048, 16, 32, 64, 0)), dict(vol=(1024, 16, 32, 64, 0), kernel=A.MTC_KERNEL_FAST),
+    dict(vol=(32, 1024, 32, 64, 1), kernel=A.MTC_KERNEL_PRECISION),
 ])
 def test_create_rejects_invalid_descriptions(kw):
     ctx = C.c_void_p()
