@@ -6,9 +6,10 @@
 //
 //  * Footprint-packed texels.  Every trilinear (bilinear) fetch of the
 //    plain kernel issues 8 (4) dword loads.  Here each texel of a derived
-//    volume stores its whole 2x2 footprint in one 16-byte entry, so an LF
-//    sample is two dwordx4 loads (z0 and z1 slices), a curl / weather / HF
-//    sample one.  Only the channels the density needs are kept, already
+//    volume stores its whole filter footprint in one entry, so every sample
+//    touches one cache line: an LF sample is two dwordx4 loads from one
+//    32-byte entry (the 2x2x2 footprint), a curl / weather / HF sample one
+//    16-byte load.  Only the channels the density needs are kept, already
 //    combined: LF -> (r, fbm = .625g + .25b + .125a) as half2 (filtering is
 //    linear, so filtering the combined channel equals combining the
 //    filtered channels, :505-507); HF -> fbm (:552-554); curl -> (r, g);
@@ -54,18 +55,17 @@ __device__ __forceinline__ float2 lerp2(float2 p, float2 q, float w) {
 }
 __device__ __forceinline__ h2v lerp2h(h2v p, h2v q, h2v w) { return __builtin_elementwise_fma(w, q - p, p); }
 
-// Footprint entry i of an array as a 32-bit unsigned byte offset from the
+// 16-byte entry i of an array as a 32-bit unsigned byte offset from the
 // array base, so the load takes the base from scalar registers and needs
-// no per-lane 64-bit address arithmetic.  Footprint arrays stay below
-// 4 GiB: volumes up to 512^3 (mtc_create rejects larger ones for these
-// kernels), maps up to 8192^2.
+// no per-lane 64-bit address arithmetic.  Footprint arrays stay within
+// 4 GiB: volumes up to 512^3 (2^27 LF entries of 32 bytes; mtc_create
+// rejects larger ones for these kernels), maps up to 8192^2.
 __device__ __forceinline__ uint4 entry(const uint4* __restrict__ q, uint32_t i) {
   return *(const uint4*)((const char*)q + (i << 4));
 }
 
 struct Coord {
-  uint32_t i0;   // linear index of the (x0, y0, z0) footprint entry
-  uint32_t i1;   // linear index of the (x0, y0, z1) entry (z wraps)
+  uint32_t i0;   // linear index of the (x0, y0, z0) texel
   float ax, ay, az;
 };
 
@@ -81,19 +81,21 @@ __device__ __forceinline__ Coord coord3(int sh, f3 s) {
   c.ay = uy - fy;
   c.az = uz - fz;
   const uint32_t x0 = (uint32_t)(int)fx & m, y0 = (uint32_t)(int)fy & m, z0 = (uint32_t)(int)fz & m;
-  const uint32_t z1 = (z0 + 1u) & m;
-  const uint32_t xy = (y0 << sh) + x0;
-  c.i0 = (z0 << (2 * sh)) + xy;
-  c.i1 = (z1 << (2 * sh)) + xy;
+  c.i0 = (((z0 << sh) + y0) << sh) + x0;
   return c;
 }
 
-// LF: (r, fbm) trilinear from two footprint entries
+// LF: (r, fbm) trilinear from the texel's 32-byte footprint entry (two
+// 16-byte halves: the z0 and z1 slices of the 2x2x2 footprint).  One cache
+// line per sample instead of two measured 17% faster for the whole march
+// than the 16-byte 2x2 (x, y) entries, although the table doubles (64 MiB
+// at 128^3): the march is limited by L2 request throughput more than by
+// capacity.
 template <bool F16>
 __device__ __forceinline__ float2 sample_lf(const uint4* __restrict__ q, int sh, f3 s) {
   Coord c = coord3(sh, s);
-  uint4 a = entry(q, c.i0);
-  uint4 b = entry(q, c.i1);
+  uint4 a = entry(q, 2u * c.i0);
+  uint4 b = entry(q, 2u * c.i0 + 1u);
   if (F16) {
     h2v wx = {(_Float16)c.ax, (_Float16)c.ax}, wy = {(_Float16)c.ay, (_Float16)c.ay},
         wz = {(_Float16)c.az, (_Float16)c.az};
@@ -895,20 +897,22 @@ __device__ __forceinline__ void texel4(const void* src, int fmt, size_t idx, flo
   }
 }
 
-// LF -> (r, .625g + .25b + .125a) at the 2x2 (x, y) footprint of each texel
+// LF -> (r, .625g + .25b + .125a) at the 2x2x2 footprint of each texel:
+// entry 2 idx holds the z0 slice's 2x2, entry 2 idx + 1 the z1 slice's
 __global__ __launch_bounds__(256) void pack_lf_kernel(const void* src, int fmt, int sh, uint4* out) {
   const int N = 1 << sh, m = N - 1;
   const size_t idx = (size_t)blockIdx.x * 256 + threadIdx.x;
   if (idx >= (size_t)N * N * N) return;
   int x = (int)(idx & m), y = (int)((idx >> sh) & m), z = (int)(idx >> (2 * sh));
-  uint32_t e[4];
-  for (int k = 0; k < 4; ++k) {
-    int xx = (x + (k & 1)) & m, yy = (y + (k >> 1)) & m;
+  uint32_t e[8];
+  for (int k = 0; k < 8; ++k) {
+    int xx = (x + (k & 1)) & m, yy = (y + ((k >> 1) & 1)) & m, zz = (z + (k >> 2)) & m;
     float v[4];
-    texel4(src, fmt, (((size_t)z << sh) + yy) * N + xx, v);
+    texel4(src, fmt, (((size_t)zz << sh) + yy) * N + xx, v);
     e[k] = pack_h2(v[0], (v[1] * 0.625f + v[2] * 0.25f) + v[3] * 0.125f);
   }
-  out[idx] = make_uint4(e[0], e[1], e[2], e[3]);
+  out[2 * idx] = make_uint4(e[0], e[1], e[2], e[3]);
+  out[2 * idx + 1] = make_uint4(e[4], e[5], e[6], e[7]);
 }
 
 // HF -> .625r + .25g + .125b at the 2x2x2 footprint

@@ -249,7 +249,7 @@ int mtc_create(const mtc_desc* d, mtc_ctx** out) {
     if (e == hipSuccess) e = hipEventCreateWithFlags(&c->fork, hipEventDisableTiming);
     if (e == hipSuccess) e = hipEventCreateWithFlags(&c->join, hipEventDisableTiming);
     const size_t lf3 = (size_t)v.lf_dim * v.lf_dim * v.lf_dim, hf3 = (size_t)v.hf_dim * v.hf_dim * v.hf_dim;
-    if (e == hipSuccess) e = hipMalloc((void**)&c->q_lf, lf3 * sizeof(uint4));
+    if (e == hipSuccess) e = hipMalloc((void**)&c->q_lf, 2 * lf3 * sizeof(uint4));  // 32-byte 2x2x2 LF entries
     if (e == hipSuccess) e = hipMalloc((void**)&c->q_hf, hf3 * sizeof(uint4));
     if (e == hipSuccess) e = hipMalloc((void**)&c->q_curl, (size_t)v.curl_dim * v.curl_dim * sizeof(uint4));
     if (e == hipSuccess) e = hipMalloc((void**)&c->q_weather, (size_t)v.weather_dim * v.weather_dim * sizeof(uint4));
