@@ -110,7 +110,7 @@ def light_net(steps: int = 4000, seed_poses: int = 3):
         params, sun = M.preset_params(PRESET)
         net = M.LightNet(capacity=1 << 24)
         rng = np.random.default_rng(seed_poses)
-        with M.CloudStage(960, 540, M.VolumeSpec(128, 32, 128, 512, A.MTC_FMT_UNORM8), kernel=A.MTC_KERNEL_FAST) as st:
+        with M.CloudStage(960, 540, M.VolumeSpec(128, 32, 128, 512, A.MTC_FMT_UNORM8), kernel=A.MTC_KERNEL_LEARNED) as st:
             st.generate_volumes(1, 2)
             for i in range(16):
                 p = np.radians(rng.uniform(8.0, 40.0))

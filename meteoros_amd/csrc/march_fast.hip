@@ -304,12 +304,15 @@ __device__ __forceinline__ void shell_span(const FrameConsts& fc, const Ray& ray
 }
 
 // Everything a step needs that is constant along the ray.
+// Per-ray state of rayMarch.  Everything the same for all rays of a frame
+// (eye, planet centre, shell offsets, wind, 1 / thickness, ...) is read
+// from the frame constants where it is used, so it stays in scalar
+// registers; values assigned in the divergent per-ray set-up would each
+// take a vector register.
 struct RayMarch {
-  Ray ray;
-  f3 c, inner, wk, woff;
-  float start_t, end_t, stepSize, cos_angle, HG_light, inv_thick, inv_div, ls, ws, cone_scale;
+  f3 d;             // ray direction (the origin is the eye)
+  float start_t, end_t, stepSize, HG_light, ls, cone_scale;
   float cos_ratio;  // (cos_angle - 0.7) / (1 - 0.7) of GetLightEnergy's first remap
-  int pixelID, cap;
 };
 
 // One primary step as seen by its lane, before the cone march.
@@ -337,20 +340,22 @@ __device__ __forceinline__ Prim eval_primary(const FrameConsts& fc, const FastVi
   o.base = 0.0f;
   o.rel = mk(0.0f, 0.0f, 0.0f);
   const float t = __fmaf_rn((float)k, rm.stepSize, rm.start_t);
-  o.valid = (t < rm.end_t) && (k < rm.cap);
+  o.valid = (t < rm.end_t) && (k < fc.mk_cap);
   if (!o.valid) return o;
-  const Ray& ray = rm.ray;
-  const int ji = ((rm.pixelID + (int)t) & 15) * 3;
-  const f3 dj = mk(ray.d.x + sjit[ji], ray.d.y + sjit[ji + 1], ray.d.z + sjit[ji + 2]);
-  const f3 pos = ray.o + t * dj;
-  const f3 rel = (pos - rm.inner) * rm.inv_thick;
-  const f3 sp = rel * rm.inv_div;
-  const f3 pe = pos - ray.o;
-  const f3 pc = pos - rm.c;
+  const int ji = ((fc.frame + (int)t) & 15) * 3;
+  const f3 dj = mk(rm.d.x + sjit[ji], rm.d.y + sjit[ji + 1], rm.d.z + sjit[ji + 2]);
+  // pos = eye + t dj; the shader's differences pos - eye, pos - inner
+  // (inner = (eye.x, inner_off, eye.z)) and pos - c (c = (eye.x, -R, eye.z))
+  // from t dj and two per-frame offsets
+  const f3 pe = t * dj;
+  const float it = fc.mk_inv_thick;
+  const f3 rel = mk(pe.x * it, (pe.y + fc.mk_rel_y0) * it, pe.z * it);
+  const f3 sp = rel * fc.mk_inv_div;
+  const f3 pc = mk(pe.x, pe.y + fc.mk_c_y0, pe.z);
   const float lr = __builtin_amdgcn_sqrtf(dot(pe, pe));  // v_sqrt_f32 (1 ulp); the plain kernel keeps IEEE
-  const float cosT = dot(ray.d, pc) * __builtin_amdgcn_rsqf(dot(pc, pc));
-  const float h = fabsf(cosT * (lr - rm.ls)) * rm.inv_thick;
-  const f3 sk = sp + h * rm.wk + rm.woff;
+  const float cosT = dot(rm.d, pc) * __builtin_amdgcn_rsqf(dot(pc, pc));
+  const float h = fabsf(cosT * (lr - rm.ls)) * it;
+  const f3 sk = sp + h * ld3(fc.mk_wk) + ld3(fc.wind_offset);
   float sc = 0.0f, st = 0.0f;
   if (COV == MTC_COVERAGE_WEATHER) {
     // getDensityHeightGradientForPoint profiles (:479-481)
@@ -368,7 +373,7 @@ __device__ __forceinline__ Prim eval_primary(const FrameConsts& fc, const FastVi
   float cov = fc.coverage_const, gw = 1.0f;
   if (COV == MTC_COVERAGE_WEATHER) {
     if (sc == 0.0f && st == 0.0f) return o;
-    const float2 w = sample_map<F16>(fv.weather, fv.weather_sh, sp.x, sp.z, rm.ws);
+    const float2 w = sample_map<F16>(fv.weather, fv.weather_sh, sp.x, sp.z, fc.weather_scale);
     cov = w.x;
     gw = gradient_weight(w.y, sc, st);
     if (gw <= cov) return o;
@@ -409,30 +414,21 @@ struct ConeLds {
 // lane whose ray has finished still takes its share of the cone work.
 // Per-ray constants of rayMarch (:565-624) for a ray that is marched.
 template <bool FP64SHELL>
-__device__ __forceinline__ void setup_march(const FrameConsts& fc, const Ray& ray, int pixelID, RayMarch& rm) {
+__device__ __forceinline__ void setup_march(const FrameConsts& fc, const Ray& ray, RayMarch& rm) {
   const float d = ray.d.y;
-    rm.ray = ray;
-    rm.c = mk(ray.o.x, -fc.earth_r, ray.o.z);
-    f3 startPos;
-    shell_span<FP64SHELL>(fc, ray, rm.c, rm.start_t, rm.end_t, startPos);
-    float maxSteps = (fc.step_mode == MTC_STEPS_FIXED)
-                         ? (float)fc.steps_max
-                         : floorf(mixf((float)fc.steps_min, (float)fc.steps_max, 1.0f - d));
-    rm.stepSize = (rm.end_t - rm.start_t) / maxSteps;
-    rm.cos_angle = dot(ray.d * __builtin_amdgcn_rsqf(dot(ray.d, ray.d)), ld3(fc.light_dir));
-    rm.HG_light = fmaxf(hg_fast(rm.cos_angle, fc.hg_g), fc.hg_si * hg_fast(rm.cos_angle, fc.hg_g2));
-    rm.cos_ratio = (rm.cos_angle - 0.7f) / (1.0f - 0.7f);
-    rm.inv_thick = 1.0f / fc.thickness;
-    rm.inv_div = 1.0f / fc.sample_divisor;
-    rm.ls = length(startPos - ray.o);
-    rm.inner = mk(rm.c.x, fc.inner_off, rm.c.z);
-    const float wsk = fc.top_offset * fc.wind_skew;
-    rm.wk = mk(fc.wind_dir[0] * wsk, fc.wind_dir[1] * wsk, fc.wind_dir[2] * wsk);
-    rm.woff = ld3(fc.wind_offset);
-    rm.ws = fc.weather_scale;
-    rm.cone_scale = rm.stepSize * rm.inv_thick;
-    rm.pixelID = pixelID;
-    rm.cap = 4 * max(fc.steps_min, fc.steps_max) + 64;
+  rm.d = ray.d;
+  const f3 c = mk(ray.o.x, -fc.earth_r, ray.o.z);
+  f3 startPos;
+  shell_span<FP64SHELL>(fc, ray, c, rm.start_t, rm.end_t, startPos);
+  float maxSteps = (fc.step_mode == MTC_STEPS_FIXED)
+                       ? (float)fc.steps_max
+                       : floorf(mixf((float)fc.steps_min, (float)fc.steps_max, 1.0f - d));
+  rm.stepSize = (rm.end_t - rm.start_t) / maxSteps;
+  const float cos_angle = dot(ray.d * __builtin_amdgcn_rsqf(dot(ray.d, ray.d)), ld3(fc.light_dir));
+  rm.HG_light = fmaxf(hg_fast(cos_angle, fc.hg_g), fc.hg_si * hg_fast(cos_angle, fc.hg_g2));
+  rm.cos_ratio = (cos_angle - 0.7f) / (1.0f - 0.7f);
+  rm.ls = length(startPos - ray.o);
+  rm.cone_scale = rm.stepSize * fc.mk_inv_thick;
 }
 
 // One density term of the cone light march (:348-369) for sample i of an
@@ -468,7 +464,7 @@ __device__ __forceinline__ float cone_term(const FrameConsts& fc, const FastView
 // The network answers the rest of the cone: -(terms 1 .. n-1).
 __device__ __forceinline__ void mlp_features(const FrameConsts& fc, const Prim& pr, const RayMarch& rm, float c0,
                                              float f[9]) {
-  const float ax = pr.rel.x * rm.ws, az = pr.rel.z * rm.ws;
+  const float ax = pr.rel.x * fc.weather_scale, az = pr.rel.z * fc.weather_scale;
   f[0] = ax - floorf(ax);
   f[1] = pr.rel.y - floorf(pr.rel.y);
   f[2] = az - floorf(az);
@@ -477,7 +473,7 @@ __device__ __forceinline__ void mlp_features(const FrameConsts& fc, const Prim& 
   f[5] = fc.light_dir[0];
   f[6] = fc.light_dir[1];
   f[7] = fc.light_dir[2];
-  f[8] = rm.ray.d.y;
+  f[8] = rm.d.y;
 }
 
 __device__ __forceinline__ uint32_t step_hash(uint32_t a, uint32_t b, uint32_t c, uint32_t seed) {
@@ -544,7 +540,7 @@ __device__ __forceinline__ void march_tile(const FrameConsts& fc, const FastView
       } else {
         n_marched = 1;
         gdone = false;
-        setup_march<FP64SHELL>(fc, ray, pixelID, rm);
+        setup_march<FP64SHELL>(fc, ray, rm);
       }
     }
   }
@@ -764,7 +760,7 @@ __device__ __forceinline__ void march_tile(const FrameConsts& fc, const FastView
 
 // One frame: a 256-lane workgroup marches an 8 x (32 / G) tile of rays.
 template <int COV, bool F16, bool FP64SHELL, bool COUNT, int G>
-__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(COUNT ? 1 : 5))) void march_kernel(
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(COUNT ? 1 : 6))) void march_kernel(
     FrameConsts fc, FastViews fv, OutputViews ov, int full) {
   __shared__ float sjit[48];
   __shared__ float skern[18];
@@ -817,7 +813,7 @@ __global__ __launch_bounds__(256) void light_samples_kernel(FrameConsts fc, Fast
   const Ray ray = cast_ray_folded(fc, ux, uy, fc.jit_march[0], fc.jit_march[1]);
   if (ray.d.y < 0.0f || ray.d.y < fc.fade_start) return;
   RayMarch rm;
-  setup_march<true>(fc, ray, fc.frame, rm);
+  setup_march<true>(fc, ray, rm);
   float accum = 0.0f;
   for (int k = 0;; ++k) {
     const Prim pr = eval_primary<COV, false>(fc, fv, sjit, rm, k);

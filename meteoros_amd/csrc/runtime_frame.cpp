@@ -10,6 +10,7 @@
 // Compiled with -ffp-contract=off.
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
 #include <cmath>
 #include <cstring>
 
@@ -208,6 +209,15 @@ int build_frame_consts(const mtc_camera* cur, const mtc_camera* old, const mtc_t
   const float sac = (float)0.999956676946448443553574619906976478926848692873900859324;
   fc->sun_disk_lo = sac;
   fc->sun_disk_hi = sac + 0.00002f;
+  // per-frame values of the optimised marches (their per-ray set-up did
+  // the same FP32 operations)
+  fc->mk_inv_thick = 1.0f / fc->thickness;
+  fc->mk_inv_div = 1.0f / fc->sample_divisor;
+  fc->mk_rel_y0 = fc->eye[1] - fc->inner_off;
+  fc->mk_c_y0 = fc->eye[1] + fc->earth_r;
+  const float wsk = fc->top_offset * fc->wind_skew;
+  for (int i = 0; i < 3; ++i) fc->mk_wk[i] = fc->wind_dir[i] * wsk;
+  fc->mk_cap = 4 * std::max(fc->steps_min, fc->steps_max) + 64;
   (void)mulv;
   return MTC_OK;
 }

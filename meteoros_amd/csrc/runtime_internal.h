@@ -55,6 +55,14 @@ struct FrameConsts {
   float rp_ro[3];               // (eye - (eye.x, -R_earth, eye.z)) / R_in
   float rp_C;                   // dot(rp_ro, rp_ro) - 1
   float rp_frac[64];            // i / motion_blur_samples
+  // optimised marches: per-frame values they would otherwise derive per
+  // ray (kept out of vector registers); same FP32 operations as the shader
+  float mk_inv_thick;   // 1 / thickness
+  float mk_inv_div;     // 1 / sample_divisor
+  float mk_rel_y0;      // eye.y - inner_off: relative height offset of the eye
+  float mk_c_y0;        // eye.y + earth_r: eye height above the planet centre
+  float mk_wk[3];       // wind_dir * (top_offset * wind_skew)
+  int32_t mk_cap;       // 4 * max(steps_min, steps_max) + 64 (loop guard)
 };
 
 // Fills FrameConsts from the boundary structs (runtime_frame.cpp).

@@ -124,7 +124,7 @@ def test_sample_generator_matches_oracle_records():
     W, H = 64, 36
     dims = (128, 32, 128, 512)
     cam, tm, sun, params = _scene(W, H, 20.0)
-    with M.CloudStage(W, H, M.VolumeSpec(*dims), kernel=A.MTC_KERNEL_FAST) as st, M.LightNet(1 << 18) as net:
+    with M.CloudStage(W, H, M.VolumeSpec(*dims), kernel=A.MTC_KERNEL_LEARNED) as st, M.LightNet(1 << 18) as net:
         st.generate_volumes(1, 2)
         lf, hf, curl, weather = st.get_volumes()
         n = net.collect(st, cam, tm, sun, params, seed=1, keep_fraction=1.0)
@@ -161,7 +161,7 @@ def _train_on_scenes(net, dims, vol_seed=(1, 2), steps=4000):
 
     W, H = 480, 270
     t0 = time.perf_counter()
-    with M.CloudStage(W, H, M.VolumeSpec(*dims), kernel=A.MTC_KERNEL_FAST) as st:
+    with M.CloudStage(W, H, M.VolumeSpec(*dims), kernel=A.MTC_KERNEL_LEARNED) as st:
         st.generate_volumes(*vol_seed)
         rng = np.random.default_rng(12)
         for i in range(12):
