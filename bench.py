@@ -154,6 +154,57 @@ def run_gpu(kernel: int, seq, warmup: int, steps: int, flags: int = 0, lanes: in
     return st, (t1 - t0)
 
 
+BATCH_SIZES = [(512, 512)] * 6 + [(128, 128)] * 40 + [(256, 144)] * 40 + [(1920, 1080)] * 10
+
+
+def batch_views(seed: int = 11):
+    """The 96-view batch: 6 cube faces at 512^2 (90 deg field of view), 40
+    probes at 128^2, 40 views at 256x144 and 10 views at 1920x1080 (24.4 M
+    pixels, every one marched); seeded camera poses and sun directions,
+    64-160 primary steps per view."""
+    rng = np.random.default_rng(seed)
+    faces = [(1.0, 0.0, 0.0), (-1.0, 0.0, 0.0), (0.0, 0.0, 1.0), (0.0, 0.0, -1.0), (1e-3, 1.0, 0.0), (1e-3, -1.0, 0.0)]
+    cams, times, suns, params = [], [], [], []
+    for k, (w, h) in enumerate(BATCH_SIZES):
+        p, sun = M.preset_params(4)
+        p.steps_min = p.steps_max = int(rng.integers(64, 161))
+        M.sun_from_angles(float(rng.uniform(10.0, 60.0)), float(rng.uniform(0.0, 360.0)), sun, p)
+        if k < 6:
+            cam = M.Camera((0.0, 0.0, 0.0), faces[k], w, h, 90.0)
+        else:
+            pitch, yaw = np.radians(rng.uniform(5.0, 50.0)), np.radians(rng.uniform(0.0, 360.0))
+            d = (float(np.cos(pitch) * np.sin(yaw)), float(np.sin(pitch)), float(-np.cos(pitch) * np.cos(yaw)))
+            cam = M.Camera((0.0, 0.0, 0.0), d, w, h, 45.0)
+        t = M.Time()
+        t.ubo.frame_count_mod16 = k % 16
+        t.ubo.time[1] = 0.25 * k
+        cams.append(cam.ubo())
+        times.append(t.ubo)
+        suns.append(sun)
+        params.append(p)
+    return cams, times, suns, params
+
+
+def run_batch(kernel: int, renders: int = 5) -> dict:
+    """Autotunes and times the 96-view batch (one persistent launch in a
+    hipGraph per render, uniforms uploaded each render)."""
+    cams, times, suns, params = batch_views()
+    with M.CloudStage(16, 16, M.VolumeSpec(128, 32, 128, 512, A.MTC_FMT_UNORM8), kernel=kernel) as st:
+        st.generate_volumes(1, 2)
+        with M.Batch(st, BATCH_SIZES) as b:
+            best, tune_ms = b.autotune(cams, times, suns, params)
+            b.render(cams, times, suns, params)
+            st.synchronize()
+            t0 = time.perf_counter()
+            for _ in range(renders):
+                b.render(cams, times, suns, params)
+            st.synchronize()
+            dt = (time.perf_counter() - t0) / renders
+    px = sum(w * h for w, h in BATCH_SIZES)
+    return {"views": len(BATCH_SIZES), "pixels": px, "tuning": best, "autotune_best_ms": round(tune_ms, 3),
+            "ms_per_batch": round(1000.0 * dt, 3), "mpixels_per_s": round(px / dt / 1e6, 1), "renders": renders}
+
+
 def count_work(seq, warmup: int, steps: int) -> dict:
     st, _ = run_gpu(A.MTC_KERNEL_PLAIN, seq, warmup, 0, flags=A.MTC_FLAG_COUNTERS)
     st.reset_counters()
@@ -239,6 +290,7 @@ def main() -> int:
                     help="Adam steps of the learned light's network (untimed set-up; fewer for profiler runs)")
     ap.add_argument("--cpu-seconds", type=float, default=12.0)
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--no-batch", action="store_true", help="skip the 96-view batch measurement")
     args = ap.parse_args()
     if args.gpus != 1:
         print(json.dumps({"error": "the cloud stage runs on one GPU; only --gpus 1 is supported",
@@ -322,6 +374,9 @@ def main() -> int:
         "roofline": roof,
         "accuracy": r["acc"],
     }
+    if not args.no_batch and best in ("fast", "precision"):
+        out["batch_96_views"] = run_batch(KERNELS[best])
+        print(f"[bench] 96-view batch: {out['batch_96_views']}", file=sys.stderr, flush=True)
     if not args.no_cpu_baseline:
         out["cpu_baseline"] = cpu_baseline(seq, args.cpu_seconds)
     print(json.dumps(out))
