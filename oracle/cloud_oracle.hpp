@@ -22,11 +22,15 @@
 //   UNORM8 texels enter as their integer code 0..255 and the filtered value
 //   is multiplied by fl(1/255); FP16 texels enter as their exact value.
 //
-// Pinning (tests/test_oracle_*.py): closed forms for Halton, remap,
-// smoothstep, EncodeFloatRGBA, HG normalisation and the sphere quadratic;
-// brute-force Vulkan-formula trilinear on tiny volumes; invariants of the
-// march (empty volume => background, ocean colour, accum in range,
-// trip counts); FP32 vs FP64 agreement.
+// Pinning (tests/test_oracle_*.py, tests/test_lighting_pins.py): closed
+// forms for Halton, remap, smoothstep, EncodeFloatRGBA, HG normalisation
+// and the sphere quadratic; brute-force Vulkan-formula trilinear on tiny
+// volumes; invariants of the march (empty volume => background, ocean
+// colour, accum in range, trip counts); the sky's scattering constants
+// recomputed from the physics, its symmetries and the Rayleigh phase
+// normalisation; GetLightEnergy's factor structure; FP32 vs FP64
+// agreement.  Parity unpinned: the sky's air-mass constants (0.15, 93.885,
+// -1.253 in atmosphere_color) beyond the FP32/FP64 twin agreement.
 #pragma once
 #include <cmath>
 #include <cstdint>
@@ -338,11 +342,16 @@ template <class T> static inline v3<T> calc_beta_r(const Frame<T>& f) {
   T sunFade = T(1) - clampT(T(1) - std::exp(f.sky_sun.y / T(450000.0)), T(0), T(1));
   return rayleigh_total<T>() * (rayleigh - T(1) + sunFade);
 }
+// MIE_CONST (:85): pi (2 pi / lambda)^(v - 2) K at lambda = 680/550/450 nm,
+// v = 4, K = (0.686, 0.678, 0.666), scaled by 1e-14 (the shader's value)
+template <class T> static inline v3<T> mie_const() {
+  return V((T)1.839991851443397, (T)2.779802391966052, (T)4.079047954386109);
+}
 template <class T> static inline v3<T> calc_beta_v() {
   T turbidity = T(10);
   T mie = T(0.005);
   T c = (T(0.2) * turbidity) * T(10E-18);
-  v3<T> mc = V((T)1.839991851443397, (T)2.779802391966052, (T)4.079047954386109);
+  v3<T> mc = mie_const<T>();
   return ((T(0.434) * c) * mc) * mie;
 }
 template <class T> static inline T calc_sun_intensity(const Frame<T>& f) {

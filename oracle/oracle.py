@@ -85,6 +85,12 @@ def lib() -> C.CDLL:
         h.orc_noise_generate.argtypes = [C.c_int, C.c_int, C.c_int, C.c_uint32, C.c_int, C.c_int, vp, C.c_int]
         h.orc_to_half.restype = C.c_uint16
         h.orc_to_half.argtypes = [C.c_float]
+        h.orc_light_energy.restype = C.c_double
+        h.orc_light_energy.argtypes = [C.c_double] * 6 + [C.c_int]
+        h.orc_rayleigh_phase.restype = C.c_double
+        h.orc_rayleigh_phase.argtypes = [C.c_double]
+        h.orc_sky_constants.restype = None
+        h.orc_sky_constants.argtypes = [P(C.c_double)]
         _lib = h
     return _lib
 
@@ -301,6 +307,23 @@ def atmosphere(sun, params, direction, origin=(0.0, 0.0, 0.0), precision=32):
     out = (C.c_double * 3)()
     lib().orc_atmosphere(C.byref(sun), C.byref(params), _d3(direction), _d3(origin), precision, out)
     return np.array(list(out))
+
+
+def light_energy(h, dl, ds, phase, cosang, brightness=1.0, precision=64) -> float:
+    return float(lib().orc_light_energy(float(h), float(dl), float(ds), float(phase), float(cosang), float(brightness),
+                                        int(precision)))
+
+
+def rayleigh_phase(c) -> float:
+    return float(lib().orc_rayleigh_phase(float(c)))
+
+
+def sky_constants():
+    """(RAYLEIGH_TOTAL, MIE_CONST) as the oracle holds them."""
+    out = (C.c_double * 6)()
+    lib().orc_sky_constants(out)
+    v = np.array(list(out))
+    return v[:3], v[3:]
 
 
 def to_half(f: float) -> int:

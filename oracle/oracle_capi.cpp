@@ -301,6 +301,23 @@ void orc_atmosphere(const mtc_sun* sun, const mtc_cloud_params* p, const double 
   }
 }
 
+// GetLightEnergy (cloudRayMarch.comp:331-388) of one sample
+double orc_light_energy(double h, double dl, double ds, double phase, double cosang, double brightness,
+                        int precision) {
+  if (precision == 64) return orc::light_energy<double>(h, dl, ds, phase, cosang, 0.0, brightness);
+  return (double)orc::light_energy<float>((float)h, (float)dl, (float)ds, (float)phase, (float)cosang, 0.0f,
+                                          (float)brightness);
+}
+// rayleighPhase (:401-404)
+double orc_rayleigh_phase(double c) { return orc::rayleigh_phase<double>(c); }
+// RAYLEIGH_TOTAL (:86) then MIE_CONST (:85) as the oracle holds them
+void orc_sky_constants(double out[6]) {
+  const auto r = orc::rayleigh_total<double>();
+  const auto m = orc::mie_const<double>();
+  out[0] = r.x; out[1] = r.y; out[2] = r.z;
+  out[3] = m.x; out[4] = m.y; out[5] = m.z;
+}
+
 // ---------------------------------------------------------------------------
 // noise generator (specification of kernel a)
 // ---------------------------------------------------------------------------

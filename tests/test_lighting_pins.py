@@ -1,0 +1,138 @@
+"""Pins of the oracle's lighting (cloudRayMarch.comp): the Preetham
+background's scattering constants against the physics they are computed
+from, its symmetries, the Rayleigh phase normalisation, and the factor
+structure of GetLightEnergy (:331-388) derived by hand from its branches.
+
+A dropped term, a swapped remap bound, a wrong exponent or a transposed
+component fails at least one of these; none of them re-evaluates the
+oracle's expressions."""
+import numpy as np
+import pytest
+
+import meteoros_amd as M
+from oracle import oracle as O
+
+LAMBDA = np.array([680e-9, 550e-9, 450e-9])  # red, green, blue (m)
+
+
+def test_rayleigh_phase_integrates_to_one():
+    # 3 / (16 pi) (1 + c^2) over the sphere; 8-point Gauss-Legendre is exact
+    # for this quadratic
+    x, w = np.polynomial.legendre.leggauss(8)
+    total = 2.0 * np.pi * sum(wi * O.rayleigh_phase(xi) for xi, wi in zip(x, w))
+    assert abs(total - 1.0) < 1e-12
+
+
+def test_rayleigh_total_from_the_scattering_formula():
+    # total Rayleigh coefficient 8 pi^3 (n^2 - 1)^2 (6 + 3 pn) / (3 N lambda^4 (6 - 7 pn)),
+    # refractive index of air n = 1.0003, N = 2.545e25 molecules / m^3,
+    # depolarisation factor pn = 0.035
+    n, N, pn = 1.0003, 2.545e25, 0.035
+    expected = 8.0 * np.pi ** 3 * (n * n - 1.0) ** 2 * (6.0 + 3.0 * pn) / (3.0 * N * LAMBDA ** 4 * (6.0 - 7.0 * pn))
+    rayleigh, _ = O.sky_constants()
+    np.testing.assert_allclose(rayleigh, expected, rtol=1e-12)
+
+
+def test_mie_constant_from_the_junge_exponent_and_k():
+    # pi (2 pi / lambda)^(v - 2) K with Junge exponent v = 4 and
+    # K = (0.686, 0.678, 0.666); the shader's MIE_CONST is that value
+    # times 1e-14 (its e14 exponent is missing, so the Mie term is ~1e-20
+    # of the Rayleigh one: kept as written)
+    v, K = 4.0, np.array([0.686, 0.678, 0.666])
+    expected = np.pi * (2.0 * np.pi / LAMBDA) ** (v - 2.0) * K * 1e-14
+    _, mie = O.sky_constants()
+    np.testing.assert_allclose(mie, expected, rtol=1e-12)
+
+
+def _rot_y(v, a):
+    c, s = np.cos(a), np.sin(a)
+    return np.array([c * v[0] + s * v[2], v[1], -s * v[0] + c * v[2]])
+
+
+def _sky(params, sun, d):
+    d = np.asarray(d, dtype=np.float64)
+    return O.atmosphere(sun, params, d / np.linalg.norm(d), (0.0, 0.0, 0.0), precision=64)
+
+
+def _with_sky_sun(params, s):
+    p = type(params).from_buffer_copy(params)
+    for i in range(3):
+        p.sky_sun_location[i] = float(s[i])
+    return p
+
+
+@pytest.mark.parametrize("angle", [0.4, 2.0, -2.9])
+def test_background_is_symmetric_about_the_vertical(angle):
+    # the sky depends on the direction only through its height (zenith
+    # angle) and its angle to the sun: turning both about the vertical
+    # leaves every colour unchanged
+    params, sun = M.preset_params(2)
+    s0 = np.array(params.sky_sun_location[:3], dtype=np.float64)
+    s1 = _rot_y(s0, angle)
+    p1 = _with_sky_sun(params, np.float32(s1))
+    s1f = np.array(p1.sky_sun_location[:3], dtype=np.float64)
+    # compare against the exactly representable rotated sun
+    p0 = _with_sky_sun(params, _rot_y(s1f, -angle))
+    for d in ([0.3, 0.4, -0.8], [-0.7, 0.1, 0.2], [0.0, 0.95, 0.1], [0.2, -0.3, 0.9]):
+        a = _sky(p0, sun, d)
+        b = _sky(p1, sun, _rot_y(np.asarray(d, float), angle))
+        np.testing.assert_allclose(b, a, rtol=1e-5)
+
+
+def test_background_is_mirror_symmetric_about_the_sun_plane():
+    # directions mirrored across the vertical plane through the sun keep
+    # their height and their angle to the sun
+    params, sun = M.preset_params(2)
+    p = _with_sky_sun(params, (0.0, 30000.0, -100000.0))  # sun in the y-z plane
+    for d in ([0.3, 0.4, -0.8], [0.9, 0.05, 0.1], [0.05, 0.9, -0.3]):
+        m = [-d[0], d[1], d[2]]
+        np.testing.assert_allclose(_sky(p, sun, m), _sky(p, sun, d), rtol=1e-13)
+    # and a direction off that plane is not symmetric to an arbitrary one
+    assert not np.allclose(_sky(p, sun, [0.3, 0.4, -0.8]), _sky(p, sun, [0.3, 0.4, 0.8]), rtol=1e-3)
+
+
+def E(h=0.5, dl=0.3, ds=0.4, phase=0.2, cosang=0.5, brightness=1.0):
+    return O.light_energy(h, dl, ds, phase, cosang, brightness, precision=64)
+
+
+@pytest.mark.parametrize("dl", [0.05, 0.4, 2.5])
+def test_light_energy_attenuation_toward_the_sun_is_beer_squared(dl):
+    # cos >= 0.7: remap(cos, 0.7, 1, s, s / 4) <= s = exp(-dl), so the max
+    # picks exp(-dl) and energy ~ exp(-dl) * exp(-dl)
+    for c in (0.7, 0.85, 1.0):
+        assert E(dl=dl, cosang=c) / E(dl=0.0, cosang=c) == pytest.approx(np.exp(-2.0 * dl), rel=1e-12)
+
+
+@pytest.mark.parametrize("c", [-1.0, 0.0, 0.3, 0.69])
+def test_light_energy_away_from_the_sun_grows_linearly_in_cos(c):
+    # cos < 0.7: the remap extrapolates, s (1 + 2.5 (0.7 - cos)) > s
+    assert E(cosang=c) / E(cosang=0.7) == pytest.approx(1.0 + 2.5 * (0.7 - c), rel=1e-12)
+    assert E(cosang=0.9) == pytest.approx(E(cosang=0.7), rel=1e-12)
+
+
+@pytest.mark.parametrize("h", [0.0, 0.3, 1.0])
+def test_light_energy_depth_term_is_a_square_root(h):
+    # remap(h / 8, 0.3, 0.85, 0.5, 2) < 0.5 for h in [0, 1], so the clamp
+    # fixes the exponent at 0.5: depth = 0.05 + sqrt(ds) at every height
+    for ds in (0.0, 0.04, 0.36, 1.0):
+        assert E(h=h, ds=ds) / E(h=h, ds=1.0) == pytest.approx((0.05 + np.sqrt(ds)) / 1.05, rel=1e-12)
+
+
+def test_light_energy_height_term_plateaus():
+    # vertical = clamp(remap(1.5 h, 0.07, 0.34, 0.1, 1), 0.1, 1)^0.8: flat
+    # at 0.1^0.8 below h = 0.07 / 1.5 and at 1 above h = 0.34 / 1.5
+    top = E(h=1.0)
+    for h in (0.0, 0.03, 0.046):
+        assert E(h=h) / top == pytest.approx(0.1 ** 0.8, rel=1e-12)
+    for h in (0.23, 0.5, 0.9):
+        assert E(h=h) == pytest.approx(top, rel=1e-12)
+    # linear in the remapped value between, to the power 0.8
+    h = 0.1
+    assert E(h=h) / top == pytest.approx((0.1 + 0.9 * (1.5 * h - 0.07) / 0.27) ** 0.8, rel=1e-12)
+
+
+def test_light_energy_is_linear_in_phase_and_brightness():
+    assert E(phase=0.4) == pytest.approx(2.0 * E(phase=0.2), rel=1e-14)
+    assert E(brightness=3.0) == pytest.approx(3.0 * E(brightness=1.0), rel=1e-14)
+    # and the FP32 twin agrees with the FP64 one
+    assert O.light_energy(0.5, 0.3, 0.4, 0.2, 0.5, 1.0, precision=32) == pytest.approx(E(), rel=1e-6)
