@@ -29,8 +29,8 @@
 // colour, accum in range, trip counts); the sky's scattering constants
 // recomputed from the physics, its symmetries and the Rayleigh phase
 // normalisation; GetLightEnergy's factor structure; FP32 vs FP64
-// agreement.  Parity unpinned: the sky's air-mass constants (0.15, 93.885,
-// -1.253 in atmosphere_color) beyond the FP32/FP64 twin agreement.
+// agreement; the air-mass term by its physical limits (1 at the zenith,
+// sec z while the atmosphere looks plane-parallel, ~36 at the horizon).
 #pragma once
 #include <cmath>
 #include <cstdint>
@@ -359,13 +359,18 @@ template <class T> static inline T calc_sun_intensity(const Frame<T>& f) {
   return T(1000.0) * std::fmax(T(0), T(1) - std::pow((T)2.718281828459,
                                                      -(((T)1.6110731557 - std::acos(zc)) / T(1.5))));
 }
+// the optical length's "inverse" (:438): Kasten's relative optical air mass
+// 1 / (cos z + 0.15 (93.885 - z_deg)^-1.253)
+template <class T> static inline T relative_air_mass(T zenith) {
+  return T(1.0) / (std::cos(zenith) + T(0.15) * std::pow(T(93.885) - ((zenith * T(180.0)) / (T)3.14159265), T(-1.253)));
+}
 template <class T> static v3<T> atmosphere_color(const Frame<T>& f, v3<T> dir, v3<T> sunDir, T sunIntensity) {
   sunDir = normalize(sunDir);
   T sunE = sunIntensity * calc_sun_intensity(f);
   v3<T> BR = calc_beta_r(f);
   v3<T> BM = calc_beta_v<T>();
   T zenith = std::acos(std::fmax(T(0), dir.y));
-  T inv = T(1.0) / (std::cos(zenith) + T(0.15) * std::pow(T(93.885) - ((zenith * T(180.0)) / (T)3.14159265), T(-1.253)));
+  T inv = relative_air_mass(zenith);
   T sR = T(8.4E3) * inv;
   T sM = T(1.25E3) * inv;
   v3<T> ex = (-BR) * sR + BM * sM;

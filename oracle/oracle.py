@@ -87,6 +87,8 @@ def lib() -> C.CDLL:
         h.orc_to_half.argtypes = [C.c_float]
         h.orc_light_energy.restype = C.c_double
         h.orc_light_energy.argtypes = [C.c_double] * 6 + [C.c_int]
+        h.orc_air_mass.restype = C.c_double
+        h.orc_air_mass.argtypes = [C.c_double]
         h.orc_rayleigh_phase.restype = C.c_double
         h.orc_rayleigh_phase.argtypes = [C.c_double]
         h.orc_sky_constants.restype = None
@@ -312,6 +314,10 @@ def atmosphere(sun, params, direction, origin=(0.0, 0.0, 0.0), precision=32):
 def light_energy(h, dl, ds, phase, cosang, brightness=1.0, precision=64) -> float:
     return float(lib().orc_light_energy(float(h), float(dl), float(ds), float(phase), float(cosang), float(brightness),
                                         int(precision)))
+
+
+def air_mass(zenith_rad) -> float:
+    return float(lib().orc_air_mass(float(zenith_rad)))
 
 
 def rayleigh_phase(c) -> float:

@@ -308,6 +308,8 @@ double orc_light_energy(double h, double dl, double ds, double phase, double cos
   return (double)orc::light_energy<float>((float)h, (float)dl, (float)ds, (float)phase, (float)cosang, 0.0f,
                                           (float)brightness);
 }
+// relative optical air mass of the sky's optical length (:437-438), FP64
+double orc_air_mass(double zenith) { return orc::relative_air_mass<double>(zenith); }
 // rayleighPhase (:401-404)
 double orc_rayleigh_phase(double c) { return orc::rayleigh_phase<double>(c); }
 // RAYLEIGH_TOTAL (:86) then MIE_CONST (:85) as the oracle holds them

@@ -44,6 +44,20 @@ def test_mie_constant_from_the_junge_exponent_and_k():
     np.testing.assert_allclose(mie, expected, rtol=1e-12)
 
 
+def test_optical_length_is_an_air_mass():
+    # the sky's optical length scales with a relative air mass: 1 by
+    # definition at the zenith, sec z while the atmosphere looks
+    # plane-parallel, finite (~36 for Kasten's fit) at the horizon, growing
+    assert O.air_mass(0.0) == pytest.approx(1.0, rel=1e-3)
+    for zdeg in (20.0, 45.0, 60.0):
+        z = np.radians(zdeg)
+        assert O.air_mass(z) * np.cos(z) == pytest.approx(1.0, rel=5e-3)
+    horizon = O.air_mass(np.pi / 2)
+    assert 30.0 < horizon < 40.0
+    zs = np.radians(np.linspace(0.0, 90.0, 91))
+    assert np.all(np.diff([O.air_mass(z) for z in zs]) > 0)
+
+
 def _rot_y(v, a):
     c, s = np.cos(a), np.sin(a)
     return np.array([c * v[0] + s * v[2], v[1], -s * v[0] + c * v[2]])
