@@ -221,7 +221,8 @@ enum {
 enum {
   MTC_FLAG_COUNTERS = 1,     /* accumulate mtc_counters with device atomics  */
   MTC_FLAG_FULL_MARCH = 2,   /* march every pixel (no 4x4 selection)          */
-  MTC_FLAG_TIMING = 4        /* hipEvents around each kernel (mtc_get_timings) */
+  MTC_FLAG_TIMING = 4        /* hipEvents around each kernel on its stream
+                                (mtc_get_timings); frames keep their structure */
 };
 
 typedef struct mtc_desc {
@@ -252,8 +253,10 @@ typedef struct mtc_counters {
                                 optical depth, in units of 1e-6       */
 } mtc_counters;
 
-/* Kernel time accumulated with hipEvents on the launch stream
- * (MTC_FLAG_TIMING), milliseconds summed over `frames` frames. */
+/* Kernel time accumulated with hipEvents on the stream each kernel runs
+ * on (MTC_FLAG_TIMING), milliseconds summed over `frames` frames.  In the
+ * optimised contexts the reprojection runs beside the march, so the two
+ * overlap and each duration includes the other's interference. */
 typedef struct mtc_timings {
   uint64_t frames;
   double reproject_ms;
@@ -315,6 +318,11 @@ int mtc_reset_counters(mtc_ctx* ctx);
  * the recorded events). */
 int mtc_get_timings(mtc_ctx* ctx, mtc_timings* out);
 int mtc_reset_timings(mtc_ctx* ctx);
+/* With MTC_FLAG_TIMING, attach the timing events to one frame in every
+ * `stride` (>= 1; default 1): the events cost each timed frame a few
+ * percent, so a long sequence can be timed on a sample of its frames.  No
+ * counterpart in the reference (a profiling control of this library). */
+int mtc_set_timing_stride(mtc_ctx* ctx, int32_t stride);
 
 /* Reprojection source-texel map for testing: for every pixel and tap the
  * linear index y*W+x of the previous-frame texel the tap reads

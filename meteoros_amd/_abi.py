@@ -158,6 +158,7 @@ SIGNATURES = [
     ("mtc_reset_counters", C.c_int, [C.c_void_p]),
     ("mtc_get_timings", C.c_int, [C.c_void_p, P(MtcTimings)]),
     ("mtc_reset_timings", C.c_int, [C.c_void_p]),
+    ("mtc_set_timing_stride", C.c_int, [C.c_void_p, C.c_int32]),
     ("mtc_reprojection_indices", C.c_int, [C.c_void_p, P(MtcCamera), P(MtcCamera), P(MtcTime), P(MtcCloudParams),
                                            C.c_void_p, C.c_size_t]),
     ("mtc_load_image_rgba8", C.c_int, [C.c_char_p, P(C.c_int32), P(C.c_int32), C.c_void_p, C.c_size_t]),

@@ -28,11 +28,15 @@
 //  * Variant 1 (precision): filtering in packed FP16 (v_pk_fma_f16), FP16
 //    volumes, the shell quadratic in FP64; transmittance and radiance stay
 //    FP32.
-// Multiply-adds contract to fma in this file (the library builds with
-// -ffp-contract=off so that the plain kernel keeps the shader's one
-// rounding per operation); the optimised kernels are held to the FP64
-// oracle, which fma approaches more closely, not to bit equality.
-#pragma clang fp contract(fast)
+// Multiply-adds within one expression contract to fma in this file (the
+// library builds with -ffp-contract=off so that the plain kernel keeps the
+// shader's one rounding per operation); the optimised kernels are held to
+// the FP64 oracle, which fma approaches more closely, not to bit equality.
+// Contraction stays within an expression ("on", not "fast"): across
+// statements it would depend on what the compiler sees between them (a
+// lane shuffle, or none when one lane marches a ray), and a ray's bits
+// must not depend on how many lanes share it.
+#pragma clang fp contract(on)
 #include "march_common.h"
 #include "mlp_common.h"
 
@@ -965,30 +969,30 @@ hipError_t pack_fast_resources(const VolumeViews& vv, uint4* lf, uint4* hf, uint
 
 template <int COV, bool F16, bool FP64, int G>
 static void launch_v(hipStream_t s, const FrameConsts& fc, const mtc::fast::FastViews& fv, const OutputViews& ov,
-                     int full) {
+                     int full, const KernelEvents& ke) {
   int rays_x = full ? fc.width : (fc.width + 3) / 4;
   int rays_y = full ? fc.height : (fc.height + 3) / 4;
   constexpr int TX = 8, TY = 256 / G / 8;
   dim3 grid((rays_x + TX - 1) / TX, (rays_y + TY - 1) / TY), block(256);
-  if (ov.counters) hipLaunchKernelGGL((mtc::fast::march_kernel<COV, F16, FP64, true, G>), grid, block, 0, s, fc, fv, ov, full);
-  else hipLaunchKernelGGL((mtc::fast::march_kernel<COV, F16, FP64, false, G>), grid, block, 0, s, fc, fv, ov, full);
+  if (ov.counters) launch_timed(mtc::fast::march_kernel<COV, F16, FP64, true, G>, grid, block, s, ke, fc, fv, ov, full);
+  else launch_timed(mtc::fast::march_kernel<COV, F16, FP64, false, G>, grid, block, s, ke, fc, fv, ov, full);
 }
 
 template <bool F16, bool FP64, int G>
 static void launch_cov(hipStream_t s, const FrameConsts& fc, const mtc::fast::FastViews& fv, const OutputViews& ov,
-                       int full) {
-  if (fc.coverage_mode == MTC_COVERAGE_WEATHER) launch_v<MTC_COVERAGE_WEATHER, F16, FP64, G>(s, fc, fv, ov, full);
-  else launch_v<MTC_COVERAGE_CONSTANT, F16, FP64, G>(s, fc, fv, ov, full);
+                       int full, const KernelEvents& ke) {
+  if (fc.coverage_mode == MTC_COVERAGE_WEATHER) launch_v<MTC_COVERAGE_WEATHER, F16, FP64, G>(s, fc, fv, ov, full, ke);
+  else launch_v<MTC_COVERAGE_CONSTANT, F16, FP64, G>(s, fc, fv, ov, full, ke);
 }
 
 template <bool F16, bool FP64>
 static bool launch_g(int lanes_per_ray, hipStream_t s, const FrameConsts& fc, const mtc::fast::FastViews& fv,
-                     const OutputViews& ov, int full) {
+                     const OutputViews& ov, int full, const KernelEvents& ke) {
   switch (lanes_per_ray) {
-    case 1: launch_cov<F16, FP64, 1>(s, fc, fv, ov, full); return true;
-    case 2: launch_cov<F16, FP64, 2>(s, fc, fv, ov, full); return true;
-    case 4: launch_cov<F16, FP64, 4>(s, fc, fv, ov, full); return true;
-    case 8: launch_cov<F16, FP64, 8>(s, fc, fv, ov, full); return true;
+    case 1: launch_cov<F16, FP64, 1>(s, fc, fv, ov, full, ke); return true;
+    case 2: launch_cov<F16, FP64, 2>(s, fc, fv, ov, full, ke); return true;
+    case 4: launch_cov<F16, FP64, 4>(s, fc, fv, ov, full, ke); return true;
+    case 8: launch_cov<F16, FP64, 8>(s, fc, fv, ov, full, ke); return true;
     default: return false;
   }
 }
@@ -997,13 +1001,13 @@ static bool launch_g(int lanes_per_ray, hipStream_t s, const FrameConsts& fc, co
 // filtering); lanes_per_ray in {1, 2, 4, 8}
 hipError_t launch_march_fast(const FrameConsts& fc, const uint4* lf, const uint4* hf, const uint4* curl,
                              const uint4* weather, const VolumeViews& vv, const OutputViews& ov, int full, int variant,
-                             int lanes_per_ray, hipStream_t stream) {
+                             int lanes_per_ray, hipStream_t stream, const KernelEvents& ke) {
   mtc::fast::FastViews fv{lf, hf, curl, weather, vv.lf_shift, vv.hf_shift, vv.curl_shift, vv.weather_shift};
   // Both variants keep the shell quadratic in FP64: in FP32 its hit points
   // move by 0.3-1 m (C = dot(rO, rO) - 1 cancels), which shifts every t_k and
   // flips the int(t) jitter index of most steps (:631).
-  bool ok = variant == 0 ? launch_g<false, true>(lanes_per_ray, stream, fc, fv, ov, full)
-                         : launch_g<true, true>(lanes_per_ray, stream, fc, fv, ov, full);
+  bool ok = variant == 0 ? launch_g<false, true>(lanes_per_ray, stream, fc, fv, ov, full, ke)
+                         : launch_g<true, true>(lanes_per_ray, stream, fc, fv, ov, full, ke);
   if (!ok) return hipErrorInvalidValue;
   return hipGetLastError();
 }
@@ -1023,29 +1027,30 @@ static mtc::fast::FastViews learned_views(const uint4* lf, const uint4* hf, cons
 
 template <int COV, int G>
 static void launch_learned_v(hipStream_t s, const FrameConsts& fc, const mtc::fast::FastViews& fv,
-                             const OutputViews& ov, int full) {
+                             const OutputViews& ov, int full,
+                             const KernelEvents& ke) {
   int rays_x = full ? fc.width : (fc.width + 3) / 4;
   int rays_y = full ? fc.height : (fc.height + 3) / 4;
   constexpr int TX = 8, TY = 256 / G / 8;
   dim3 grid((rays_x + TX - 1) / TX, (rays_y + TY - 1) / TY), block(256);
   if (ov.counters)
-    hipLaunchKernelGGL((mtc::fast::march_learned_kernel<COV, true, G>), grid, block, 0, s, fc, fv, ov, full);
+    launch_timed(mtc::fast::march_learned_kernel<COV, true, G>, grid, block, s, ke, fc, fv, ov, full);
   else
-    hipLaunchKernelGGL((mtc::fast::march_learned_kernel<COV, false, G>), grid, block, 0, s, fc, fv, ov, full);
+    launch_timed(mtc::fast::march_learned_kernel<COV, false, G>, grid, block, s, ke, fc, fv, ov, full);
 }
 
 // Learned-light frame (fast sampling, FP64 shell); lanes_per_ray 4 or 8.
 hipError_t launch_march_learned(const FrameConsts& fc, const uint4* lf, const uint4* hf, const uint4* curl,
                                 const uint4* weather, const VolumeViews& vv, const OutputViews& ov, int full,
-                                int lanes_per_ray, const LearnedLight& ll, hipStream_t s) {
+                                int lanes_per_ray, const LearnedLight& ll, hipStream_t s, const KernelEvents& ke) {
   const mtc::fast::FastViews fv = learned_views(lf, hf, curl, weather, vv, ll);
   const bool wx = fc.coverage_mode == MTC_COVERAGE_WEATHER;
   if (lanes_per_ray == 8) {
-    if (wx) launch_learned_v<MTC_COVERAGE_WEATHER, 8>(s, fc, fv, ov, full);
-    else launch_learned_v<MTC_COVERAGE_CONSTANT, 8>(s, fc, fv, ov, full);
+    if (wx) launch_learned_v<MTC_COVERAGE_WEATHER, 8>(s, fc, fv, ov, full, ke);
+    else launch_learned_v<MTC_COVERAGE_CONSTANT, 8>(s, fc, fv, ov, full, ke);
   } else if (lanes_per_ray == 4) {
-    if (wx) launch_learned_v<MTC_COVERAGE_WEATHER, 4>(s, fc, fv, ov, full);
-    else launch_learned_v<MTC_COVERAGE_CONSTANT, 4>(s, fc, fv, ov, full);
+    if (wx) launch_learned_v<MTC_COVERAGE_WEATHER, 4>(s, fc, fv, ov, full, ke);
+    else launch_learned_v<MTC_COVERAGE_CONSTANT, 4>(s, fc, fv, ov, full, ke);
   } else {
     return hipErrorInvalidValue;
   }

@@ -210,18 +210,18 @@ __global__ __launch_bounds__(256) void march_kernel(FrameConsts fc, VolumeViews 
 
 // Host launcher: grid of 16x16-ray workgroups over the marched-ray lattice.
 hipError_t launch_march_plain(const FrameConsts& fc, const VolumeViews& vv, const OutputViews& ov, int full,
-                              hipStream_t stream) {
+                              hipStream_t stream, const KernelEvents& ke) {
   int rays_x = full ? fc.width : (fc.width + 3) / 4;
   int rays_y = full ? fc.height : (fc.height + 3) / 4;
   dim3 grid((rays_x + 15) / 16, (rays_y + 15) / 16);
   dim3 block(256);
   const bool count = ov.counters != nullptr;
   if (vv.format == MTC_FMT_FP16) {
-    if (count) hipLaunchKernelGGL((mtc::plain::march_kernel<MTC_FMT_FP16, true>), grid, block, 0, stream, fc, vv, ov, full);
-    else hipLaunchKernelGGL((mtc::plain::march_kernel<MTC_FMT_FP16, false>), grid, block, 0, stream, fc, vv, ov, full);
+    if (count) launch_timed(mtc::plain::march_kernel<MTC_FMT_FP16, true>, grid, block, stream, ke, fc, vv, ov, full);
+    else launch_timed(mtc::plain::march_kernel<MTC_FMT_FP16, false>, grid, block, stream, ke, fc, vv, ov, full);
   } else {
-    if (count) hipLaunchKernelGGL((mtc::plain::march_kernel<MTC_FMT_UNORM8, true>), grid, block, 0, stream, fc, vv, ov, full);
-    else hipLaunchKernelGGL((mtc::plain::march_kernel<MTC_FMT_UNORM8, false>), grid, block, 0, stream, fc, vv, ov, full);
+    if (count) launch_timed(mtc::plain::march_kernel<MTC_FMT_UNORM8, true>, grid, block, stream, ke, fc, vv, ov, full);
+    else launch_timed(mtc::plain::march_kernel<MTC_FMT_UNORM8, false>, grid, block, stream, ke, fc, vv, ov, full);
   }
   return hipGetLastError();
 }

@@ -25,11 +25,12 @@ __global__ __launch_bounds__(256) void reproject_kernel(FrameConsts fc, OutputVi
 
 // mode 0: the update; 1: the tap-index map; 2: the update without the
 // pixels the march of this frame writes
-hipError_t launch_reproject(const FrameConsts& fc, const OutputViews& ov, int mode, hipStream_t stream) {
+hipError_t launch_reproject(const FrameConsts& fc, const OutputViews& ov, int mode, hipStream_t stream,
+                            const KernelEvents& ke) {
   dim3 grid((fc.width + 63) / 64, (fc.height + 3) / 4);
   dim3 block(256);
-  if (mode == 1) hipLaunchKernelGGL((mtc::reproj::reproject_kernel<true, false>), grid, block, 0, stream, fc, ov);
-  else if (mode == 2) hipLaunchKernelGGL((mtc::reproj::reproject_kernel<false, true>), grid, block, 0, stream, fc, ov);
-  else hipLaunchKernelGGL((mtc::reproj::reproject_kernel<false, false>), grid, block, 0, stream, fc, ov);
+  if (mode == 1) launch_timed(mtc::reproj::reproject_kernel<true, false>, grid, block, stream, ke, fc, ov);
+  else if (mode == 2) launch_timed(mtc::reproj::reproject_kernel<false, true>, grid, block, stream, ke, fc, ov);
+  else launch_timed(mtc::reproj::reproject_kernel<false, false>, grid, block, stream, ke, fc, ov);
   return hipGetLastError();
 }

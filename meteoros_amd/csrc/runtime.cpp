@@ -25,16 +25,17 @@
 #include "runtime_internal.h"
 
 hipError_t launch_march_plain(const FrameConsts& fc, const VolumeViews& vv, const OutputViews& ov, int full,
-                              hipStream_t stream);
+                              hipStream_t stream, const KernelEvents& ke);
 hipError_t launch_march_fast(const FrameConsts& fc, const uint4* lf, const uint4* hf, const uint4* curl,
                              const uint4* weather, const VolumeViews& vv, const OutputViews& ov, int full, int variant,
-                             int lanes_per_ray, hipStream_t stream);
+                             int lanes_per_ray, hipStream_t stream, const KernelEvents& ke);
 hipError_t launch_march_learned(const FrameConsts& fc, const uint4* lf, const uint4* hf, const uint4* curl,
                                 const uint4* weather, const VolumeViews& vv, const OutputViews& ov, int full,
-                                int lanes_per_ray, const LearnedLight& ll, hipStream_t s);
+                                int lanes_per_ray, const LearnedLight& ll, hipStream_t s, const KernelEvents& ke);
 hipError_t pack_fast_resources(const VolumeViews& vv, uint4* lf, uint4* hf, uint4* curl, uint4* weather,
                                hipStream_t s);
-hipError_t launch_reproject(const FrameConsts& fc, const OutputViews& ov, int mode, hipStream_t stream);
+hipError_t launch_reproject(const FrameConsts& fc, const OutputViews& ov, int mode, hipStream_t stream,
+                            const KernelEvents& ke = KernelEvents());
 hipError_t launch_noise(int kind, void* out, int N, int fmt, uint32_t seed, hipStream_t stream);
 
 namespace {
@@ -77,16 +78,19 @@ hipEvent_t take_event(mtc_ctx* c) {
   return e;
 }
 
-// Fold the oldest `keep`-exceeding pending frames into the totals.
+// Fold the oldest `keep`-exceeding pending frames into the totals.  A
+// frame's events: reprojection start / end (on the stream it ran on, null
+// for a full march), march start / end (on the launch stream).
 hipError_t fold_timings(mtc_ctx* c, size_t keep) {
   while (c->pending.size() > keep) {
     auto ev = c->pending.front();
     c->pending.erase(c->pending.begin());
-    hipError_t e = hipEventSynchronize(ev[2]);
+    hipError_t e = hipEventSynchronize(ev[3]);
+    if (e == hipSuccess && ev[1]) e = hipEventSynchronize(ev[1]);
     if (e != hipSuccess) return e;
     float a = 0.0f, b = 0.0f;
     if (ev[0] && ev[1]) (void)hipEventElapsedTime(&a, ev[0], ev[1]);
-    (void)hipEventElapsedTime(&b, ev[1], ev[2]);
+    (void)hipEventElapsedTime(&b, ev[2], ev[3]);
     c->totals.frames += 1;
     c->totals.reproject_ms += a;
     c->totals.march_ms += b;
@@ -138,22 +142,24 @@ int check_ctx(mtc_ctx* c, const char* fn) {
   return MTC_OK;
 }
 
-int launch_march(mtc_ctx* c, const FrameConsts& fc, const OutputViews& ov, int full, hipStream_t s) {
+int launch_march(mtc_ctx* c, const FrameConsts& fc, const OutputViews& ov, int full, hipStream_t s,
+                 const KernelEvents& ke = KernelEvents()) {
   VolumeViews vv = volume_views(c);
   hipError_t e;
   switch (c->desc.kernel) {
-    case MTC_KERNEL_PLAIN: e = launch_march_plain(fc, vv, ov, full, s); break;
+    case MTC_KERNEL_PLAIN: e = launch_march_plain(fc, vv, ov, full, s, ke); break;
     case MTC_KERNEL_FAST:
-      e = launch_march_fast(fc, c->q_lf, c->q_hf, c->q_curl, c->q_weather, vv, ov, full, 0, c->lanes_per_ray, s);
+      e = launch_march_fast(fc, c->q_lf, c->q_hf, c->q_curl, c->q_weather, vv, ov, full, 0, c->lanes_per_ray, s, ke);
       break;
     case MTC_KERNEL_PRECISION:
-      e = launch_march_fast(fc, c->q_lf, c->q_hf, c->q_curl, c->q_weather, vv, ov, full, 1, c->lanes_per_ray, s);
+      e = launch_march_fast(fc, c->q_lf, c->q_hf, c->q_curl, c->q_weather, vv, ov, full, 1, c->lanes_per_ray, s, ke);
       break;
     case MTC_KERNEL_LEARNED: {
       LearnedLight ll;
       int rc = learned_light_of(c, &ll);
       if (rc) return rc;
-      e = launch_march_learned(fc, c->q_lf, c->q_hf, c->q_curl, c->q_weather, vv, ov, full, c->lanes_per_ray, ll, s);
+      e = launch_march_learned(fc, c->q_lf, c->q_hf, c->q_curl, c->q_weather, vv, ov, full, c->lanes_per_ray, ll, s,
+                               ke);
       break;
     }
     default: return mtc_fail(MTC_ERR_UNSUPPORTED, "march: kernel variant not available in this build");
@@ -341,38 +347,31 @@ int mtc_frame(mtc_ctx* c, const mtc_camera* cur, const mtc_camera* old, const mt
   ov.counters = (c->desc.flags & MTC_FLAG_COUNTERS) ? c->counters : nullptr;
   ov.reproj_idx = nullptr;
   const int full = (c->desc.flags & MTC_FLAG_FULL_MARCH) ? 1 : 0;
-  const bool timing = (c->desc.flags & MTC_FLAG_TIMING) != 0;
-  std::array<hipEvent_t, 3> ev = {nullptr, nullptr, nullptr};
+  const bool timing = (c->desc.flags & MTC_FLAG_TIMING) != 0 && (c->frames_seen++ % (uint64_t)c->timing_stride) == 0;
+  // timing: start / end events attached to each kernel's dispatch; the
+  // frame keeps its structure (the concurrent reprojection stays
+  // concurrent) and no marker sits between the kernels
+  std::array<hipEvent_t, 4> ev = {nullptr, nullptr, nullptr, nullptr};
   if (timing) {
     HIP_TRY(fold_timings(c, 255), "mtc_frame: timing events");
-    for (int k = full ? 1 : 0; k < 3; ++k) {
+    for (int k = full ? 2 : 0; k < 4; ++k) {
       ev[k] = take_event(c);
       if (!ev[k]) return mtc_fail(MTC_ERR_HIP, "mtc_frame: hipEventCreate failed");
     }
   }
-  if (!full && !timing && c->aux) {
+  if (!full && c->aux) {
     // reprojection of the other 15/16 of the pixels beside the march
     HIP_TRY(hipEventRecord(c->fork, s), "mtc_frame: fork");
     HIP_TRY(hipStreamWaitEvent(c->aux, c->fork, 0), "mtc_frame: fork");
-    HIP_TRY(launch_reproject(fc, ov, 2, c->aux), "reprojection kernel launch");
+    HIP_TRY(launch_reproject(fc, ov, 2, c->aux, KernelEvents{ev[0], ev[1]}), "reprojection kernel launch");
     HIP_TRY(hipEventRecord(c->join, c->aux), "mtc_frame: join");
-    rc = launch_march(c, fc, ov, full, s);
-    if (rc) return rc;
-    HIP_TRY(hipStreamWaitEvent(s, c->join, 0), "mtc_frame: join");
-    c->latest = dst;
-    return MTC_OK;
+  } else if (!full) {
+    HIP_TRY(launch_reproject(fc, ov, 0, s, KernelEvents{ev[0], ev[1]}), "reprojection kernel launch");
   }
-  if (!full) {
-    if (timing) HIP_TRY(hipEventRecord(ev[0], s), "mtc_frame: event");
-    HIP_TRY(launch_reproject(fc, ov, 0, s), "reprojection kernel launch");
-  }
-  if (timing) HIP_TRY(hipEventRecord(ev[1], s), "mtc_frame: event");
-  rc = launch_march(c, fc, ov, full, s);
+  rc = launch_march(c, fc, ov, full, s, KernelEvents{ev[2], ev[3]});
   if (rc) return rc;
-  if (timing) {
-    HIP_TRY(hipEventRecord(ev[2], s), "mtc_frame: event");
-    c->pending.push_back(ev);
-  }
+  if (!full && c->aux) HIP_TRY(hipStreamWaitEvent(s, c->join, 0), "mtc_frame: join");
+  if (timing) c->pending.push_back(ev);
   c->latest = dst;
   return MTC_OK;
 }
@@ -488,6 +487,15 @@ int mtc_reset_timings(mtc_ctx* c) {
   if (rc) return rc;
   HIP_TRY(fold_timings(c, 0), "mtc_reset_timings");
   c->totals = mtc_timings{0, 0.0, 0.0};
+  c->frames_seen = 0;
+  return MTC_OK;
+}
+
+int mtc_set_timing_stride(mtc_ctx* c, int32_t stride) {
+  if (!c) return mtc_fail(MTC_ERR_INVALID_ARG, "mtc_set_timing_stride: null context");
+  if (stride < 1) return mtc_fail(MTC_ERR_INVALID_ARG, "mtc_set_timing_stride: stride must be >= 1");
+  c->timing_stride = stride;
+  c->frames_seen = 0;
   return MTC_OK;
 }
 

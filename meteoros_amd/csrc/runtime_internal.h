@@ -1,5 +1,6 @@
 // runtime_internal.h -- shared declarations of the host runtime.
 #pragma once
+#include <hip/hip_ext.h>
 #include <hip/hip_runtime.h>
 
 #include <array>
@@ -64,6 +65,20 @@ struct FrameConsts {
   float mk_wk[3];       // wind_dir * (top_offset * wind_skew)
   int32_t mk_cap;       // 4 * max(steps_min, steps_max) + 64 (loop guard)
 };
+
+// Timing events attached to one kernel dispatch (MTC_FLAG_TIMING): the
+// runtime records the kernel's start and end with the dispatch itself
+// (hipExtLaunchKernelGGL), not with separate markers that would stall the
+// queue between kernels.  Both null: an ordinary launch.
+struct KernelEvents {
+  hipEvent_t start = nullptr, stop = nullptr;
+};
+
+template <class K, class... Args>
+inline void launch_timed(K kernel, dim3 grid, dim3 block, hipStream_t s, const KernelEvents& ke, Args... args) {
+  if (ke.start || ke.stop) hipExtLaunchKernelGGL(kernel, grid, block, 0, s, ke.start, ke.stop, 0u, args...);
+  else hipLaunchKernelGGL(kernel, grid, block, 0, s, args...);
+}
 
 // Fills FrameConsts from the boundary structs (runtime_frame.cpp).
 int build_frame_consts(const mtc_camera* cur, const mtc_camera* old, const mtc_time* time, const mtc_sun* sun,
@@ -160,9 +175,11 @@ struct mtc_ctx {
   bool volumes_ready = false;
   // MTC_FLAG_TIMING: event triples (before reprojection, between, after
   // march) of frames not yet folded into the totals
-  std::vector<std::array<hipEvent_t, 3>> pending;
+  std::vector<std::array<hipEvent_t, 4>> pending;
   std::vector<hipEvent_t> event_pool;
   mtc_timings totals = {0, 0.0, 0.0};
+  int32_t timing_stride = 1;   // events on one frame in timing_stride
+  uint64_t frames_seen = 0;    // frames since creation (for the stride)
   // post-process: decoded god-ray mask footprints ((W+1) x (H+1) float4)
   // and the display image (RGBA8 or RGBA32F), allocated on first use
   float4* ray_fp = nullptr;

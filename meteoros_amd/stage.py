@@ -211,6 +211,10 @@ class CloudStage:
     def reset_timings(self) -> None:
         check(lib().mtc_reset_timings(self._ctx), "mtc_reset_timings")
 
+    def set_timing_stride(self, stride: int) -> None:
+        """With MTC_FLAG_TIMING: time one frame in `stride`."""
+        check(lib().mtc_set_timing_stride(self._ctx, int(stride)), "mtc_set_timing_stride")
+
     # -- post-process passes (postProcess_GodRays.frag, postProcess_ToneMap.frag)
     def god_rays(self, cam: A.MtcCamera, sun: A.MtcSun, stream: int | None = None) -> None:
         check(lib().mtc_god_rays(self._ctx, C.byref(cam), C.byref(sun), stream), "mtc_god_rays")

@@ -80,6 +80,7 @@ def test_create_without_device_is_an_error_not_a_crash():
 def test_null_context_calls_are_rejected():
     lib = M.lib()
     assert lib.mtc_synchronize(None) == A.MTC_ERR_INVALID_ARG
+    assert lib.mtc_set_timing_stride(None, 4) == A.MTC_ERR_INVALID_ARG
     assert lib.mtc_read_output(None, None, None, 0) == A.MTC_ERR_INVALID_ARG
     assert lib.mtc_frame(None, None, None, None, None, None, None) == A.MTC_ERR_INVALID_ARG
     assert lib.mtc_destroy(None) == A.MTC_OK

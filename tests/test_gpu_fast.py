@@ -135,3 +135,34 @@ def test_precision_variant_4k_fp16_volumes_over_poses():
         worst = min(worst, p)
     st.close()
     assert worst >= PSNR_DB
+
+
+@pytest.mark.parametrize("kernel", [A.MTC_KERNEL_FAST, A.MTC_KERNEL_PRECISION])
+def test_results_do_not_depend_on_lanes_per_ray(kernel):
+    """Each lane group replays its steps' accumulation in step order, so a
+    ray's colour, mask and the counters are the same bits whether 1, 2, 4 or
+    8 lanes share it (the grouping only changes which lane computes which
+    step and how the cone work is dealt out)."""
+    W, H = 480, 270
+    params, sun = M.preset_params(2)
+    cam = ground_camera(W, H, 15.0, 30.0).ubo()
+    tm = _time(7)
+    vol = None
+    outs = []
+    for lanes in (1, 2, 4, 8):
+        st = M.CloudStage(W, H, M.VolumeSpec(128, 32, 128, 512), kernel=kernel, flags=A.MTC_FLAG_COUNTERS,
+                          lanes_per_ray=lanes)
+        if vol is None:
+            st.generate_volumes(1, 2)
+            vol = st.get_volumes()
+        else:
+            st.set_volumes(*vol)
+        st.frame(cam, cam, tm, sun, params)
+        rgba, mask = st.read_output()
+        outs.append((rgba, mask, st.counters()))
+        st.close()
+    bits = lambda a: np.ascontiguousarray(a).view(np.uint32)  # noqa: E731
+    for rgba, mask, cnt in outs[1:]:
+        assert np.array_equal(bits(rgba), bits(outs[0][0]))
+        assert np.array_equal(bits(mask), bits(outs[0][1]))
+        assert cnt == outs[0][2]

@@ -235,3 +235,36 @@ def test_reprojection_of_the_fast_path_is_bit_identical(yaw, pitch):
     assert np.array_equal(outs[0][~sel], outs[1][~sel])
     ref, _, _ = O.frame(volumes(*SMALL), cur, old, tm, sun, params, W, H, prev=hist, precision=32)
     assert np.array_equal(outs[1][~sel], ref[~sel])
+
+
+@pytest.mark.parametrize("kernel", [A.MTC_KERNEL_PLAIN, A.MTC_KERNEL_PRECISION])
+def test_timing_events_sample_frames_without_changing_results(kernel):
+    """MTC_FLAG_TIMING attaches events to one frame in `stride`; the frames
+    it times and the ones it does not give the same pixels as an untimed
+    context."""
+    W, H = 160, 96
+    params, sun = M.preset_params(1)
+    cam = ground_camera(W, H, 25.0)
+    seq = []
+    old = cam.ubo()
+    for f in range(8):
+        cam.rotate_about_up(0.5)
+        t = _time(f)
+        seq.append((cam.ubo(), old, t.ubo))
+        old = seq[-1][0]
+    outs = []
+    for flags in (0, A.MTC_FLAG_TIMING):
+        st, _ = _stage(W, H, dims=SMALL, vol=volumes(*SMALL), kernel=kernel, flags=flags)
+        with st:
+            if flags:
+                st.set_timing_stride(3)
+            for cur, prev, tm in seq:
+                st.frame(cur, prev, tm, sun, params)
+            outs.append(st.read_output()[0])
+            if flags:
+                tt = st.timings()
+                assert tt["frames"] == 3  # frames 0, 3, 6
+                assert tt["march_ms"] > 0.0 and tt["reproject_ms"] > 0.0
+                with pytest.raises(M.MtcError):
+                    st.set_timing_stride(0)
+    assert np.array_equal(outs[0].view(np.uint32), outs[1].view(np.uint32))
