@@ -762,17 +762,54 @@ __device__ __forceinline__ void march_tile(const FrameConsts& fc, const FastView
   }
 }
 
-// One frame: a 256-lane workgroup marches an 8 x (32 / G) tile of rays.
+// The tile that workgroup r of an n-tile launch marches: the r-th of the
+// previous march's tiles taken longest level first, or tile r when slot
+// `rd` does not hold exactly one filing of every tile of this tiling.  The
+// counts and the tile are wave-uniform (scalar loads).
+__device__ __forceinline__ uint32_t ordered_tile(const TileOrder& to, uint32_t r, uint32_t n) {
+  if (!to.rd || n > to.cap) return r;
+  uint32_t total = 0;
+#pragma unroll
+  for (int l = 0; l < kOrderLevels; ++l) total += to.rd[l];
+  if (total != n) return r;
+  uint32_t base = 0;
+  for (int l = kOrderLevels - 1; l >= 0; --l) {
+    const uint32_t cnt = to.rd[l];
+    if (r < base + cnt) return to.rd[kOrderLevels + (size_t)l * to.cap + (r - base)];
+    base += cnt;
+  }
+  return r;
+}
+
+// Files tile `tile` under the level of the workgroup's duration since t0
+// (100 MHz ticks; level 2 log2(ticks / 64), so 0.64 us .. 116 us over the
+// 16 levels) and clears the counts of the slot after next (workgroup 0).
+// One lane per workgroup writes, with vector atomics and stores.
+__device__ __forceinline__ void file_tile(const TileOrder& to, uint32_t tile, uint32_t n, uint64_t t0) {
+  if (!to.wr) return;
+  if (blockIdx.x == 0 && threadIdx.x < kOrderLevels) to.clr[threadIdx.x] = 0u;
+  if (threadIdx.x != 0 || n > to.cap) return;
+  const float ticks = (float)(__builtin_amdgcn_s_memrealtime() - t0);
+  const int level = ticks < 64.0f ? 0 : min(kOrderLevels - 1, (int)(2.0f * __log2f(ticks * (1.0f / 64.0f))));
+  const uint32_t pos = atomicAdd(to.wr + level, 1u);
+  if (pos < to.cap) to.wr[kOrderLevels + (size_t)level * to.cap + pos] = tile;
+}
+
+// One frame: a one-wave workgroup marches an 8 x (8 / G) tile of rays (a
+// finer grain than four-wave groups: the launch drains in smaller steps).
+// Tiles go out in raster order: dealing them longest first (as the learned
+// kernel does) scatters the resident waves over the image, and the lost
+// texel locality costs more than the shorter tail gains.
 template <int COV, bool F16, bool FP64SHELL, bool COUNT, int G>
-__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(COUNT ? 1 : 6))) void march_kernel(
+__global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(COUNT ? 1 : 6))) void march_kernel(
     FrameConsts fc, FastViews fv, OutputViews ov, int full) {
   __shared__ float sjit[48];
   __shared__ float skern[18];
-  __shared__ ConeLds cone_lds[4];
+  __shared__ ConeLds cone_lds[1];
   if (threadIdx.x < 48) sjit[threadIdx.x] = (&fc.step_jitter[0][0])[threadIdx.x];
   if (threadIdx.x < 18) skern[threadIdx.x] = (&fc.kern[0][0])[threadIdx.x];
   __syncthreads();
-  constexpr int TX = 8, TY = 256 / G / 8;
+  constexpr int TX = 8, TY = 64 / G / 8;
   march_tile<COV, F16, FP64SHELL, COUNT, G>(fc, fv, ov.color, ov.mask, ov.counters, full, blockIdx.x * TX,
                                             blockIdx.y * TY, TX, sjit, skern, cone_lds);
 }
@@ -780,17 +817,24 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(COUNT ? 1 :
 // One frame with learned light (fast variant's sampling): the in-cloud
 // steps of each chunk query the network on MFMA before the cone march.
 template <int COV, bool COUNT, int G>
-__global__ __launch_bounds__(256) void march_learned_kernel(FrameConsts fc, FastViews fv, OutputViews ov, int full) {
+__global__ __launch_bounds__(256) void march_learned_kernel(FrameConsts fc, FastViews fv, OutputViews ov, int full,
+                                                            TileOrder to, int tiles_x) {
   __shared__ float sjit[48];
   __shared__ float skern[18];
   __shared__ ConeLds cone_lds[4];
   __shared__ __align__(16) uint16_t mlp_lds[4 * 64 * 16];
+  const uint64_t t0 = __builtin_amdgcn_s_memrealtime();
   if (threadIdx.x < 48) sjit[threadIdx.x] = (&fc.step_jitter[0][0])[threadIdx.x];
   if (threadIdx.x < 18) skern[threadIdx.x] = (&fc.kern[0][0])[threadIdx.x];
   __syncthreads();
   constexpr int TX = 8, TY = 256 / G / 8;
-  march_tile<COV, false, true, COUNT, G, true>(fc, fv, ov.color, ov.mask, ov.counters, full, blockIdx.x * TX,
-                                               blockIdx.y * TY, TX, sjit, skern, cone_lds, mlp_lds);
+  const uint32_t tile = ordered_tile(to, blockIdx.x, gridDim.x);
+  march_tile<COV, false, true, COUNT, G, true>(fc, fv, ov.color, ov.mask, ov.counters, full,
+                                               (int)(tile % (uint32_t)tiles_x) * TX,
+                                               (int)(tile / (uint32_t)tiles_x) * TY, TX, sjit, skern, cone_lds,
+                                               mlp_lds);
+  __syncthreads();  // the workgroup's duration is its slowest wave's
+  file_tile(to, tile, gridDim.x, t0);
 }
 
 // Training samples for the learned light: every pixel of the view is
@@ -970,10 +1014,10 @@ hipError_t pack_fast_resources(const VolumeViews& vv, uint4* lf, uint4* hf, uint
 template <int COV, bool F16, bool FP64, int G>
 static void launch_v(hipStream_t s, const FrameConsts& fc, const mtc::fast::FastViews& fv, const OutputViews& ov,
                      int full, const KernelEvents& ke) {
-  int rays_x = full ? fc.width : (fc.width + 3) / 4;
-  int rays_y = full ? fc.height : (fc.height + 3) / 4;
-  constexpr int TX = 8, TY = 256 / G / 8;
-  dim3 grid((rays_x + TX - 1) / TX, (rays_y + TY - 1) / TY), block(256);
+  const int rays_x = full ? fc.width : (fc.width + 3) / 4;
+  const int rays_y = full ? fc.height : (fc.height + 3) / 4;
+  constexpr int TX = 8, TY = 64 / G / 8;
+  dim3 grid((rays_x + TX - 1) / TX, (rays_y + TY - 1) / TY), block(64);
   if (ov.counters) launch_timed(mtc::fast::march_kernel<COV, F16, FP64, true, G>, grid, block, s, ke, fc, fv, ov, full);
   else launch_timed(mtc::fast::march_kernel<COV, F16, FP64, false, G>, grid, block, s, ke, fc, fv, ov, full);
 }
@@ -1027,30 +1071,32 @@ static mtc::fast::FastViews learned_views(const uint4* lf, const uint4* hf, cons
 
 template <int COV, int G>
 static void launch_learned_v(hipStream_t s, const FrameConsts& fc, const mtc::fast::FastViews& fv,
-                             const OutputViews& ov, int full,
+                             const OutputViews& ov, int full, const TileOrder& to,
                              const KernelEvents& ke) {
-  int rays_x = full ? fc.width : (fc.width + 3) / 4;
-  int rays_y = full ? fc.height : (fc.height + 3) / 4;
+  const int rays_x = full ? fc.width : (fc.width + 3) / 4;
+  const int rays_y = full ? fc.height : (fc.height + 3) / 4;
   constexpr int TX = 8, TY = 256 / G / 8;
-  dim3 grid((rays_x + TX - 1) / TX, (rays_y + TY - 1) / TY), block(256);
+  const int tiles_x = (rays_x + TX - 1) / TX, tiles_y = (rays_y + TY - 1) / TY;
+  dim3 grid((unsigned)(tiles_x * tiles_y)), block(256);
   if (ov.counters)
-    launch_timed(mtc::fast::march_learned_kernel<COV, true, G>, grid, block, s, ke, fc, fv, ov, full);
+    launch_timed(mtc::fast::march_learned_kernel<COV, true, G>, grid, block, s, ke, fc, fv, ov, full, to, tiles_x);
   else
-    launch_timed(mtc::fast::march_learned_kernel<COV, false, G>, grid, block, s, ke, fc, fv, ov, full);
+    launch_timed(mtc::fast::march_learned_kernel<COV, false, G>, grid, block, s, ke, fc, fv, ov, full, to, tiles_x);
 }
 
 // Learned-light frame (fast sampling, FP64 shell); lanes_per_ray 4 or 8.
 hipError_t launch_march_learned(const FrameConsts& fc, const uint4* lf, const uint4* hf, const uint4* curl,
                                 const uint4* weather, const VolumeViews& vv, const OutputViews& ov, int full,
-                                int lanes_per_ray, const LearnedLight& ll, hipStream_t s, const KernelEvents& ke) {
+                                int lanes_per_ray, const LearnedLight& ll, const TileOrder& to, hipStream_t s,
+                                const KernelEvents& ke) {
   const mtc::fast::FastViews fv = learned_views(lf, hf, curl, weather, vv, ll);
   const bool wx = fc.coverage_mode == MTC_COVERAGE_WEATHER;
   if (lanes_per_ray == 8) {
-    if (wx) launch_learned_v<MTC_COVERAGE_WEATHER, 8>(s, fc, fv, ov, full, ke);
-    else launch_learned_v<MTC_COVERAGE_CONSTANT, 8>(s, fc, fv, ov, full, ke);
+    if (wx) launch_learned_v<MTC_COVERAGE_WEATHER, 8>(s, fc, fv, ov, full, to, ke);
+    else launch_learned_v<MTC_COVERAGE_CONSTANT, 8>(s, fc, fv, ov, full, to, ke);
   } else if (lanes_per_ray == 4) {
-    if (wx) launch_learned_v<MTC_COVERAGE_WEATHER, 4>(s, fc, fv, ov, full, ke);
-    else launch_learned_v<MTC_COVERAGE_CONSTANT, 4>(s, fc, fv, ov, full, ke);
+    if (wx) launch_learned_v<MTC_COVERAGE_WEATHER, 4>(s, fc, fv, ov, full, to, ke);
+    else launch_learned_v<MTC_COVERAGE_CONSTANT, 4>(s, fc, fv, ov, full, to, ke);
   } else {
     return hipErrorInvalidValue;
   }

@@ -31,7 +31,8 @@ hipError_t launch_march_fast(const FrameConsts& fc, const uint4* lf, const uint4
                              int lanes_per_ray, hipStream_t stream, const KernelEvents& ke);
 hipError_t launch_march_learned(const FrameConsts& fc, const uint4* lf, const uint4* hf, const uint4* curl,
                                 const uint4* weather, const VolumeViews& vv, const OutputViews& ov, int full,
-                                int lanes_per_ray, const LearnedLight& ll, hipStream_t s, const KernelEvents& ke);
+                                int lanes_per_ray, const LearnedLight& ll, const TileOrder& to, hipStream_t s,
+                                const KernelEvents& ke);
 hipError_t pack_fast_resources(const VolumeViews& vv, uint4* lf, uint4* hf, uint4* curl, uint4* weather,
                                hipStream_t s);
 hipError_t launch_reproject(const FrameConsts& fc, const OutputViews& ov, int mode, hipStream_t stream,
@@ -121,6 +122,7 @@ void release(mtc_ctx* c) {
   (void)hipFree(c->q_hf);
   (void)hipFree(c->q_curl);
   (void)hipFree(c->q_weather);
+  (void)hipFree(c->order);
   (void)hipFree(c->ray_fp);
   (void)hipFree(c->display);
   (void)hipFree(c->txaa[0]);
@@ -142,6 +144,20 @@ int check_ctx(mtc_ctx* c, const char* fn) {
   return MTC_OK;
 }
 
+// The tile-order slots of the next learned march (read the previous
+// march's filing, file into the next slot, clear the one after).
+TileOrder next_tile_order(mtc_ctx* c) {
+  TileOrder to;
+  if (!c->order) return to;
+  const size_t words = order_slot_words(c->order_cap);
+  const uint64_t k = c->order_seq++;
+  to.rd = c->order + ((k + 2) % 3) * words;
+  to.wr = c->order + (k % 3) * words;
+  to.clr = c->order + ((k + 1) % 3) * words;
+  to.cap = c->order_cap;
+  return to;
+}
+
 int launch_march(mtc_ctx* c, const FrameConsts& fc, const OutputViews& ov, int full, hipStream_t s,
                  const KernelEvents& ke = KernelEvents()) {
   VolumeViews vv = volume_views(c);
@@ -158,8 +174,8 @@ int launch_march(mtc_ctx* c, const FrameConsts& fc, const OutputViews& ov, int f
       LearnedLight ll;
       int rc = learned_light_of(c, &ll);
       if (rc) return rc;
-      e = launch_march_learned(fc, c->q_lf, c->q_hf, c->q_curl, c->q_weather, vv, ov, full, c->lanes_per_ray, ll, s,
-                               ke);
+      e = launch_march_learned(fc, c->q_lf, c->q_hf, c->q_curl, c->q_weather, vv, ov, full, c->lanes_per_ray, ll,
+                               next_tile_order(c), s, ke);
       break;
     }
     default: return mtc_fail(MTC_ERR_UNSUPPORTED, "march: kernel variant not available in this build");
@@ -259,6 +275,16 @@ int mtc_create(const mtc_desc* d, mtc_ctx** out) {
     if (e == hipSuccess) e = hipMalloc((void**)&c->q_hf, hf3 * sizeof(uint4));
     if (e == hipSuccess) e = hipMalloc((void**)&c->q_curl, (size_t)v.curl_dim * v.curl_dim * sizeof(uint4));
     if (e == hipSuccess) e = hipMalloc((void**)&c->q_weather, (size_t)v.weather_dim * v.weather_dim * sizeof(uint4));
+    if (d->kernel == MTC_KERNEL_LEARNED) {
+      // tile order of the learned march, sized for the 4x4 frames' tiles of
+      // 8 x 32/G rays (full frames have more tiles and go unordered)
+      const uint32_t rays_x = (uint32_t)(d->width + 3) / 4, rays_y = (uint32_t)(d->height + 3) / 4;
+      const uint32_t ty = 32u / (uint32_t)c->lanes_per_ray;
+      c->order_cap = ((rays_x + 7u) / 8u) * ((rays_y + ty - 1u) / ty);
+      const size_t order_bytes = 3 * order_slot_words(c->order_cap) * sizeof(uint32_t);
+      if (e == hipSuccess) e = hipMalloc((void**)&c->order, order_bytes);
+      if (e == hipSuccess) e = hipMemsetAsync(c->order, 0, order_bytes, c->stream);
+    }
   }
   if (e == hipSuccess) e = hipMemsetAsync(c->img[0], 0, px * sizeof(float4), c->stream);
   if (e == hipSuccess) e = hipMemsetAsync(c->img[1], 0, px * sizeof(float4), c->stream);

@@ -80,6 +80,31 @@ inline void launch_timed(K kernel, dim3 grid, dim3 block, hipStream_t s, const K
   else hipLaunchKernelGGL(kernel, grid, block, 0, s, args...);
 }
 
+// Longest-first dispatch of the learned-light march.  A march's tiles
+// differ in cost by ~100x (sky rows a few us, horizon rows 100+ us per
+// wave); dealt in raster order the long ones start late and the end of the
+// launch runs on a draining chip.  Each workgroup of a learned march
+// therefore files its tile under the level of its measured duration
+// (kOrderLevels half-octave levels of the 100 MHz real-time clock) in slot
+// `wr`; the next learned march deals workgroup r the r-th tile of slot `rd`
+// taken longest level first, and clears slot `clr`'s counts for the march
+// after it.  The three slots rotate per march.  A slot is used only when
+// its counts add up to exactly the launch's tile count, which holds only
+// after one complete march of the same tiling filed every tile once, so the
+// mapping is always a permutation; otherwise (first frame, full frames)
+// workgroup r marches tile r.  The order only changes which workgroup
+// marches which tile when, never a result.  (The fast and precision
+// marches keep raster order: for them, resident waves scattered over the
+// image lose more texel-cache locality than the shorter tail gains.)
+constexpr int kOrderLevels = 16;
+struct TileOrder {
+  const uint32_t* rd = nullptr;  // [kOrderLevels counts][kOrderLevels x cap tiles]
+  uint32_t* wr = nullptr;
+  uint32_t* clr = nullptr;
+  uint32_t cap = 0;              // tiles per level (= the largest ordered tiling)
+};
+constexpr size_t order_slot_words(uint32_t cap) { return kOrderLevels + (size_t)kOrderLevels * cap; }
+
 // Fills FrameConsts from the boundary structs (runtime_frame.cpp).
 int build_frame_consts(const mtc_camera* cur, const mtc_camera* old, const mtc_time* time, const mtc_sun* sun,
                        const mtc_cloud_params* p, int width, int height, FrameConsts* fc);
@@ -170,6 +195,11 @@ struct mtc_ctx {
   uint4* q_hf = nullptr;
   uint4* q_curl = nullptr;
   uint4* q_weather = nullptr;
+  // MTC_KERNEL_LEARNED: longest-first tile order, 3 slots of
+  // order_slot_words(order_cap) words rotated by order_seq
+  uint32_t* order = nullptr;
+  uint32_t order_cap = 0;
+  uint64_t order_seq = 0;
   int latest = 0;
   int lanes_per_ray = 8;  // fast kernels: lanes cooperating on one ray
   bool volumes_ready = false;

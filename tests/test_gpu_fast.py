@@ -166,3 +166,58 @@ def test_results_do_not_depend_on_lanes_per_ray(kernel):
         assert np.array_equal(bits(rgba), bits(outs[0][0]))
         assert np.array_equal(bits(mask), bits(outs[0][1]))
         assert cnt == outs[0][2]
+
+
+@pytest.mark.parametrize("kernel", [A.MTC_KERNEL_FAST, A.MTC_KERNEL_PRECISION, A.MTC_KERNEL_LEARNED])
+def test_march_history_changes_no_pixel(kernel):
+    """Whatever a context keeps from march to march (the learned march deals
+    its tiles longest first by the previous march's measured durations; the
+    others keep raster order), each frame of a moving sequence, with a full
+    march in the middle, must give the same bits as the same frame run as
+    the first march of a fresh context from the same history."""
+    W, H = 480, 270
+    params, sun = M.preset_params(2)
+    cam = ground_camera(W, H, 12.0, 40.0)
+    ops, old = [], cam.ubo()
+    for f in range(7):
+        cam.rotate_about_up(0.5)
+        ops.append(("full" if f == 3 else "frame", cam.ubo(), old, _time(f)))
+        old = ops[-1][1]
+    spec = M.VolumeSpec(128, 32, 128, 512)
+    with M.LightNet(capacity=1 << 10) as net:
+        net.init(5)  # an untrained network answers as deterministically as a trained one
+        a = M.CloudStage(W, H, spec, kernel=kernel)
+        if kernel == A.MTC_KERNEL_LEARNED:
+            net.attach(a)
+        a.generate_volumes(1, 2)
+        vol = a.get_volumes()
+        history, results = [], []
+        for op, cur, prev, tm in ops:
+            history.append(a.read_output()[0])
+            if op == "full":
+                a.march_full(cur, tm, sun, params)
+            else:
+                a.frame(cur, prev, tm, sun, params)
+            results.append(a.read_output())
+        a.close()
+        bits = lambda x: np.ascontiguousarray(x).view(np.uint32)  # noqa: E731
+        for k, (op, cur, prev, tm) in enumerate(ops):
+            if k == 0:
+                continue  # the first march of both contexts
+            with M.CloudStage(W, H, spec, kernel=kernel) as b:
+                if kernel == A.MTC_KERNEL_LEARNED:
+                    net.attach(b)
+                b.set_volumes(*vol)
+                b.set_history(history[k])
+                if op == "full":
+                    b.march_full(cur, tm, sun, params)
+                else:
+                    b.frame(cur, prev, tm, sun, params)
+                rgba, mask = b.read_output()
+            assert np.array_equal(bits(rgba), bits(results[k][0])), f"{op} {k}: colour differs"
+            if op == "full":
+                assert np.array_equal(bits(mask), bits(results[k][1])), f"{op} {k}: mask differs"
+            else:
+                xy = O.select_pixels(W, H, tm.frame_count_mod16)
+                m0, m1 = mask[xy[:, 1], xy[:, 0]], results[k][1][xy[:, 1], xy[:, 0]]
+                assert np.array_equal(bits(m0), bits(m1)), f"{op} {k}: mask differs"
