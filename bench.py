@@ -340,6 +340,8 @@ def main() -> int:
     roof = {"bound": "mfma", "pipe": "valu-fp32", "kernel": "march", "achieved": round(achieved, 3),
             "peak": 157.3, "unit": "TFLOP/s", "frac": round(achieved / 157.3, 4), "traffic": hbm_traffic(best),
             "flops_per_launch": flops_per_launch, "avg_launch_ms": round(march_ms, 4)}
+    if best in ("fast", "precision"):
+        roof["launch"] = "march tiles + the frame's reprojection strips in one launch (its duration covers both)"
     reproj_bytes = W * H * 16 * 2
     src = results.get("plain", r)["kt"]
     reproj_alone_ms = src["reproject_ms"] / max(src["frames"], 1)
@@ -367,7 +369,8 @@ def main() -> int:
         "ray_steps_per_frame": work["primary_steps"] / args.steps,
         "speedup_vs_plain": (plain_s / dt) if plain_s else None,
         "kernels": per_kernel,
-        "kernel_ms": {"march": round(march_ms, 4), "reproject": round(reproj_ms, 4)},
+        "kernel_ms": {"march": round(march_ms, 4),
+                      "reproject": round(reproj_ms, 4) if reproj_ms > 0 else "in the march launch"},
         "reproject_roofline": {"bound": "hbm", "achieved": round(reproj_bytes / (reproj_alone_ms * 1e-3) / 1e9, 1),
                                "peak": 8000.0, "unit": "GB/s",
                                "frac": round(reproj_bytes / (reproj_alone_ms * 1e-3) / 1e9 / 8000.0, 4),

@@ -39,6 +39,7 @@
 #pragma clang fp contract(on)
 #include "march_common.h"
 #include "mlp_common.h"
+#include "reproject_pixel.h"
 
 typedef _Float16 h2v __attribute__((ext_vector_type(2)));
 
@@ -800,9 +801,25 @@ __device__ __forceinline__ void file_tile(const TileOrder& to, uint32_t tile, ui
 // Tiles go out in raster order: dealing them longest first (as the learned
 // kernel does) scatters the resident waves over the image, and the lost
 // texel locality costs more than the shorter tail gains.
+//
+// Workgroups n_tiles.. of the launch carry the frame's reprojection of the
+// other 15/16 of the pixels instead (64 x 4 pixel strips; reproject_pixel.h,
+// bit-exact with the reprojection kernel): dealt after every march tile,
+// they run in the wave slots the draining march leaves free, and the frame
+// needs no second stream and no fork / join events.  The two parts write
+// disjoint pixels and the march reads nothing the reprojection writes.
 template <int COV, bool F16, bool FP64SHELL, bool COUNT, int G>
 __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(COUNT ? 1 : 6))) void march_kernel(
-    FrameConsts fc, FastViews fv, OutputViews ov, int full) {
+    FrameConsts fc, FastViews fv, OutputViews ov, int full, int tiles_x, int n_tiles) {
+  const int b = (int)blockIdx.x;
+  if (b >= n_tiles) {
+    const int strips_x = (fc.width + 63) / 64, r = b - n_tiles;
+    const int x = (r % strips_x) * 64 + (int)threadIdx.x, y0 = (r / strips_x) * 4;
+    for (int y = y0; y < y0 + 4; ++y)
+      if (x < fc.width && y < fc.height && !mtc::reproj::marched_this_frame(fc, x, y))
+        mtc::reproj::reproject_pixel<false>(fc, ov, x, y);
+    return;
+  }
   __shared__ float sjit[48];
   __shared__ float skern[18];
   __shared__ ConeLds cone_lds[1];
@@ -810,8 +827,8 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(COUNT ? 1 : 
   if (threadIdx.x < 18) skern[threadIdx.x] = (&fc.kern[0][0])[threadIdx.x];
   __syncthreads();
   constexpr int TX = 8, TY = 64 / G / 8;
-  march_tile<COV, F16, FP64SHELL, COUNT, G>(fc, fv, ov.color, ov.mask, ov.counters, full, blockIdx.x * TX,
-                                            blockIdx.y * TY, TX, sjit, skern, cone_lds);
+  march_tile<COV, F16, FP64SHELL, COUNT, G>(fc, fv, ov.color, ov.mask, ov.counters, full, (b % tiles_x) * TX,
+                                            (b / tiles_x) * TY, TX, sjit, skern, cone_lds);
 }
 
 // One frame with learned light (fast variant's sampling): the in-cloud
@@ -1013,45 +1030,54 @@ hipError_t pack_fast_resources(const VolumeViews& vv, uint4* lf, uint4* hf, uint
 
 template <int COV, bool F16, bool FP64, int G>
 static void launch_v(hipStream_t s, const FrameConsts& fc, const mtc::fast::FastViews& fv, const OutputViews& ov,
-                     int full, const KernelEvents& ke) {
+                     int full, bool with_reproj, const KernelEvents& ke) {
   const int rays_x = full ? fc.width : (fc.width + 3) / 4;
   const int rays_y = full ? fc.height : (fc.height + 3) / 4;
   constexpr int TX = 8, TY = 64 / G / 8;
-  dim3 grid((rays_x + TX - 1) / TX, (rays_y + TY - 1) / TY), block(64);
-  if (ov.counters) launch_timed(mtc::fast::march_kernel<COV, F16, FP64, true, G>, grid, block, s, ke, fc, fv, ov, full);
-  else launch_timed(mtc::fast::march_kernel<COV, F16, FP64, false, G>, grid, block, s, ke, fc, fv, ov, full);
+  const int tiles_x = (rays_x + TX - 1) / TX, n_tiles = tiles_x * ((rays_y + TY - 1) / TY);
+  const int strips = with_reproj ? ((fc.width + 63) / 64) * ((fc.height + 3) / 4) : 0;
+  dim3 grid((unsigned)(n_tiles + strips)), block(64);
+  if (ov.counters)
+    launch_timed(mtc::fast::march_kernel<COV, F16, FP64, true, G>, grid, block, s, ke, fc, fv, ov, full, tiles_x,
+                 n_tiles);
+  else
+    launch_timed(mtc::fast::march_kernel<COV, F16, FP64, false, G>, grid, block, s, ke, fc, fv, ov, full, tiles_x,
+                 n_tiles);
 }
 
 template <bool F16, bool FP64, int G>
 static void launch_cov(hipStream_t s, const FrameConsts& fc, const mtc::fast::FastViews& fv, const OutputViews& ov,
-                       int full, const KernelEvents& ke) {
-  if (fc.coverage_mode == MTC_COVERAGE_WEATHER) launch_v<MTC_COVERAGE_WEATHER, F16, FP64, G>(s, fc, fv, ov, full, ke);
-  else launch_v<MTC_COVERAGE_CONSTANT, F16, FP64, G>(s, fc, fv, ov, full, ke);
+                       int full, bool rp, const KernelEvents& ke) {
+  if (fc.coverage_mode == MTC_COVERAGE_WEATHER) launch_v<MTC_COVERAGE_WEATHER, F16, FP64, G>(s, fc, fv, ov, full, rp, ke);
+  else launch_v<MTC_COVERAGE_CONSTANT, F16, FP64, G>(s, fc, fv, ov, full, rp, ke);
 }
 
 template <bool F16, bool FP64>
 static bool launch_g(int lanes_per_ray, hipStream_t s, const FrameConsts& fc, const mtc::fast::FastViews& fv,
-                     const OutputViews& ov, int full, const KernelEvents& ke) {
+                     const OutputViews& ov, int full, bool rp, const KernelEvents& ke) {
   switch (lanes_per_ray) {
-    case 1: launch_cov<F16, FP64, 1>(s, fc, fv, ov, full, ke); return true;
-    case 2: launch_cov<F16, FP64, 2>(s, fc, fv, ov, full, ke); return true;
-    case 4: launch_cov<F16, FP64, 4>(s, fc, fv, ov, full, ke); return true;
-    case 8: launch_cov<F16, FP64, 8>(s, fc, fv, ov, full, ke); return true;
+    case 1: launch_cov<F16, FP64, 1>(s, fc, fv, ov, full, rp, ke); return true;
+    case 2: launch_cov<F16, FP64, 2>(s, fc, fv, ov, full, rp, ke); return true;
+    case 4: launch_cov<F16, FP64, 4>(s, fc, fv, ov, full, rp, ke); return true;
+    case 8: launch_cov<F16, FP64, 8>(s, fc, fv, ov, full, rp, ke); return true;
     default: return false;
   }
 }
 
 // variant 0 = fast (FP32 filtering), 1 = precision (packed FP16
-// filtering); lanes_per_ray in {1, 2, 4, 8}
+// filtering); lanes_per_ray in {1, 2, 4, 8}; with_reproj (4x4 frames
+// only): the launch also carries the reprojection of the other pixels
+// (ov.prev -> ov.color)
 hipError_t launch_march_fast(const FrameConsts& fc, const uint4* lf, const uint4* hf, const uint4* curl,
                              const uint4* weather, const VolumeViews& vv, const OutputViews& ov, int full, int variant,
-                             int lanes_per_ray, hipStream_t stream, const KernelEvents& ke) {
+                             int lanes_per_ray, bool with_reproj, hipStream_t stream, const KernelEvents& ke) {
+  if (with_reproj && (full || !ov.prev)) return hipErrorInvalidValue;
   mtc::fast::FastViews fv{lf, hf, curl, weather, vv.lf_shift, vv.hf_shift, vv.curl_shift, vv.weather_shift};
   // Both variants keep the shell quadratic in FP64: in FP32 its hit points
   // move by 0.3-1 m (C = dot(rO, rO) - 1 cancels), which shifts every t_k and
   // flips the int(t) jitter index of most steps (:631).
-  bool ok = variant == 0 ? launch_g<false, true>(lanes_per_ray, stream, fc, fv, ov, full, ke)
-                         : launch_g<true, true>(lanes_per_ray, stream, fc, fv, ov, full, ke);
+  bool ok = variant == 0 ? launch_g<false, true>(lanes_per_ray, stream, fc, fv, ov, full, with_reproj, ke)
+                         : launch_g<true, true>(lanes_per_ray, stream, fc, fv, ov, full, with_reproj, ke);
   if (!ok) return hipErrorInvalidValue;
   return hipGetLastError();
 }
