@@ -4,10 +4,14 @@
 // :1428-1447) and the texture side of Sky (Sky.cpp:25-58):
 //   * the context owns every device buffer: the four noise resources, the
 //     two RGBA32F result images (ping-pong) and the RGBA32F god-ray mask;
-//   * a frame enqueues reprojection(prev -> cur) then the march of the
-//     selected pixels into cur, in stream order (the reference recorded
-//     both dispatches with no barrier between them, a write-after-write
-//     race; here the order is explicit), then swaps the images;
+//   * a frame enqueues reprojection(prev -> cur) and the march of the
+//     selected pixels into cur, then swaps the images.  The reference
+//     recorded both dispatches with no barrier between them, a
+//     write-after-write race; here the order is explicit: plain frames run
+//     the two kernels in stream order, fast and precision frames run one
+//     launch whose reprojection leaves the marched pixels out, learned
+//     frames run the reprojection beside the march on a second stream,
+//     leaving the marched pixels out and joined back before the frame ends;
 //   * uniforms are taken by value at enqueue time (kernel arguments), so
 //     the caller may overwrite its structs as soon as the call returns.
 // Errors are status codes plus a per-thread message; nothing throws across

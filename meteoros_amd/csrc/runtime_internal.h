@@ -179,7 +179,7 @@ struct LearnedLight {
 struct mtc_ctx {
   mtc_desc desc;
   hipStream_t stream = nullptr;
-  // optimised kernels: the reprojection runs on `aux` beside the march
+  // learned-light kernel: the reprojection runs on `aux` beside the march
   // (fork / join events), the two writing disjoint pixels
   hipStream_t aux = nullptr;
   hipEvent_t fork = nullptr, join = nullptr;
