@@ -351,7 +351,7 @@ def main() -> int:
         reproj_roof = {"bound": "hbm", "achieved": round(gbs, 1), "peak": 8000.0, "unit": "GB/s",
                        "frac": round(gbs / 8000.0, 4), "avg_launch_ms": round(reproj_alone_ms, 4),
                        "measured": "in the plain context's frames, where the reprojection runs alone on the launch "
-                                   "stream (the optimised contexts carry it in the march launch or beside it)"}
+                                   "stream (the optimised contexts carry it in the march launch)"}
     per_kernel = {}
     for n in names:
         rr = results[n]

@@ -254,11 +254,9 @@ typedef struct mtc_counters {
 } mtc_counters;
 
 /* Kernel time accumulated with hipEvents on the stream each kernel runs
- * on (MTC_FLAG_TIMING), milliseconds summed over `frames` frames.  Fast
- * and precision contexts run a frame's reprojection inside the march
- * launch: march_ms covers both and reproject_ms stays 0.  Learned-light
- * contexts run it beside the march on a second stream: the two overlap and
- * each duration includes the other's interference. */
+ * on (MTC_FLAG_TIMING), milliseconds summed over `frames` frames.  The
+ * fast, precision and learned-light contexts run a frame's reprojection
+ * inside the march launch: march_ms covers both and reproject_ms stays 0. */
 typedef struct mtc_timings {
   uint64_t frames;
   double reproject_ms;

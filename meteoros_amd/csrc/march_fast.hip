@@ -833,9 +833,20 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(COUNT ? 1 : 
 
 // One frame with learned light (fast variant's sampling): the in-cloud
 // steps of each chunk query the network on MFMA before the cone march.
+// Tiles are dealt longest first (ordered_tile / file_tile); workgroups
+// n_tiles.. carry the frame's reprojection as 64 x 4 pixel strips, one row
+// per wave, as in the fast launch.
 template <int COV, bool COUNT, int G>
 __global__ __launch_bounds__(256) void march_learned_kernel(FrameConsts fc, FastViews fv, OutputViews ov, int full,
-                                                            TileOrder to, int tiles_x) {
+                                                            TileOrder to, int tiles_x, int n_tiles) {
+  const int b = (int)blockIdx.x;
+  if (b >= n_tiles) {
+    const int strips_x = (fc.width + 63) / 64, r = b - n_tiles;
+    const int x = (r % strips_x) * 64 + (int)(threadIdx.x & 63), y = (r / strips_x) * 4 + (int)(threadIdx.x >> 6);
+    if (x < fc.width && y < fc.height && !mtc::reproj::marched_this_frame(fc, x, y))
+      mtc::reproj::reproject_pixel<false>(fc, ov, x, y);
+    return;
+  }
   __shared__ float sjit[48];
   __shared__ float skern[18];
   __shared__ ConeLds cone_lds[4];
@@ -845,13 +856,13 @@ __global__ __launch_bounds__(256) void march_learned_kernel(FrameConsts fc, Fast
   if (threadIdx.x < 18) skern[threadIdx.x] = (&fc.kern[0][0])[threadIdx.x];
   __syncthreads();
   constexpr int TX = 8, TY = 256 / G / 8;
-  const uint32_t tile = ordered_tile(to, blockIdx.x, gridDim.x);
+  const uint32_t tile = ordered_tile(to, (uint32_t)b, (uint32_t)n_tiles);
   march_tile<COV, false, true, COUNT, G, true>(fc, fv, ov.color, ov.mask, ov.counters, full,
                                                (int)(tile % (uint32_t)tiles_x) * TX,
                                                (int)(tile / (uint32_t)tiles_x) * TY, TX, sjit, skern, cone_lds,
                                                mlp_lds);
   __syncthreads();  // the workgroup's duration is its slowest wave's
-  file_tile(to, tile, gridDim.x, t0);
+  file_tile(to, tile, (uint32_t)n_tiles, t0);
 }
 
 // Training samples for the learned light: every pixel of the view is
@@ -1097,32 +1108,38 @@ static mtc::fast::FastViews learned_views(const uint4* lf, const uint4* hf, cons
 
 template <int COV, int G>
 static void launch_learned_v(hipStream_t s, const FrameConsts& fc, const mtc::fast::FastViews& fv,
-                             const OutputViews& ov, int full, const TileOrder& to,
+                             const OutputViews& ov, int full, bool with_reproj, const TileOrder& to,
                              const KernelEvents& ke) {
   const int rays_x = full ? fc.width : (fc.width + 3) / 4;
   const int rays_y = full ? fc.height : (fc.height + 3) / 4;
   constexpr int TX = 8, TY = 256 / G / 8;
-  const int tiles_x = (rays_x + TX - 1) / TX, tiles_y = (rays_y + TY - 1) / TY;
-  dim3 grid((unsigned)(tiles_x * tiles_y)), block(256);
+  const int tiles_x = (rays_x + TX - 1) / TX, n_tiles = tiles_x * ((rays_y + TY - 1) / TY);
+  const int strips = with_reproj ? ((fc.width + 63) / 64) * ((fc.height + 3) / 4) : 0;
+  dim3 grid((unsigned)(n_tiles + strips)), block(256);
   if (ov.counters)
-    launch_timed(mtc::fast::march_learned_kernel<COV, true, G>, grid, block, s, ke, fc, fv, ov, full, to, tiles_x);
+    launch_timed(mtc::fast::march_learned_kernel<COV, true, G>, grid, block, s, ke, fc, fv, ov, full, to, tiles_x,
+                 n_tiles);
   else
-    launch_timed(mtc::fast::march_learned_kernel<COV, false, G>, grid, block, s, ke, fc, fv, ov, full, to, tiles_x);
+    launch_timed(mtc::fast::march_learned_kernel<COV, false, G>, grid, block, s, ke, fc, fv, ov, full, to, tiles_x,
+                 n_tiles);
 }
 
-// Learned-light frame (fast sampling, FP64 shell); lanes_per_ray 4 or 8.
+// Learned-light frame (fast sampling, FP64 shell); lanes_per_ray 4 or 8;
+// with_reproj as for launch_march_fast.
 hipError_t launch_march_learned(const FrameConsts& fc, const uint4* lf, const uint4* hf, const uint4* curl,
                                 const uint4* weather, const VolumeViews& vv, const OutputViews& ov, int full,
-                                int lanes_per_ray, const LearnedLight& ll, const TileOrder& to, hipStream_t s,
-                                const KernelEvents& ke) {
+                                int lanes_per_ray, const LearnedLight& ll, bool with_reproj, const TileOrder& to,
+                                hipStream_t s, const KernelEvents& ke) {
+  if (with_reproj && (full || !ov.prev)) return hipErrorInvalidValue;
   const mtc::fast::FastViews fv = learned_views(lf, hf, curl, weather, vv, ll);
   const bool wx = fc.coverage_mode == MTC_COVERAGE_WEATHER;
+  const bool rp = with_reproj;
   if (lanes_per_ray == 8) {
-    if (wx) launch_learned_v<MTC_COVERAGE_WEATHER, 8>(s, fc, fv, ov, full, to, ke);
-    else launch_learned_v<MTC_COVERAGE_CONSTANT, 8>(s, fc, fv, ov, full, to, ke);
+    if (wx) launch_learned_v<MTC_COVERAGE_WEATHER, 8>(s, fc, fv, ov, full, rp, to, ke);
+    else launch_learned_v<MTC_COVERAGE_CONSTANT, 8>(s, fc, fv, ov, full, rp, to, ke);
   } else if (lanes_per_ray == 4) {
-    if (wx) launch_learned_v<MTC_COVERAGE_WEATHER, 4>(s, fc, fv, ov, full, to, ke);
-    else launch_learned_v<MTC_COVERAGE_CONSTANT, 4>(s, fc, fv, ov, full, to, ke);
+    if (wx) launch_learned_v<MTC_COVERAGE_WEATHER, 4>(s, fc, fv, ov, full, rp, to, ke);
+    else launch_learned_v<MTC_COVERAGE_CONSTANT, 4>(s, fc, fv, ov, full, rp, to, ke);
   } else {
     return hipErrorInvalidValue;
   }

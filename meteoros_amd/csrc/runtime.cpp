@@ -8,10 +8,8 @@
 //     selected pixels into cur, then swaps the images.  The reference
 //     recorded both dispatches with no barrier between them, a
 //     write-after-write race; here the order is explicit: plain frames run
-//     the two kernels in stream order, fast and precision frames run one
-//     launch whose reprojection leaves the marched pixels out, learned
-//     frames run the reprojection beside the march on a second stream,
-//     leaving the marched pixels out and joined back before the frame ends;
+//     the two kernels in stream order, the optimised frames one launch
+//     whose reprojection leaves the marched pixels out;
 //   * uniforms are taken by value at enqueue time (kernel arguments), so
 //     the caller may overwrite its structs as soon as the call returns.
 // Errors are status codes plus a per-thread message; nothing throws across
@@ -35,8 +33,8 @@ hipError_t launch_march_fast(const FrameConsts& fc, const uint4* lf, const uint4
                              int lanes_per_ray, bool with_reproj, hipStream_t stream, const KernelEvents& ke);
 hipError_t launch_march_learned(const FrameConsts& fc, const uint4* lf, const uint4* hf, const uint4* curl,
                                 const uint4* weather, const VolumeViews& vv, const OutputViews& ov, int full,
-                                int lanes_per_ray, const LearnedLight& ll, const TileOrder& to, hipStream_t s,
-                                const KernelEvents& ke);
+                                int lanes_per_ray, const LearnedLight& ll, bool with_reproj, const TileOrder& to,
+                                hipStream_t s, const KernelEvents& ke);
 hipError_t pack_fast_resources(const VolumeViews& vv, uint4* lf, uint4* hf, uint4* curl, uint4* weather,
                                hipStream_t s);
 hipError_t launch_reproject(const FrameConsts& fc, const OutputViews& ov, int mode, hipStream_t stream,
@@ -131,9 +129,6 @@ void release(mtc_ctx* c) {
   (void)hipFree(c->display);
   (void)hipFree(c->txaa[0]);
   (void)hipFree(c->txaa[1]);
-  if (c->fork) (void)hipEventDestroy(c->fork);
-  if (c->join) (void)hipEventDestroy(c->join);
-  if (c->aux) (void)hipStreamDestroy(c->aux);
   if (c->stream) (void)hipStreamDestroy(c->stream);
   delete c;
 }
@@ -162,12 +157,12 @@ TileOrder next_tile_order(mtc_ctx* c) {
   return to;
 }
 
-// with_reproj: the fast / precision march launch also carries the
-// reprojection of the frame's other pixels
+// with_reproj: the optimised march launch also carries the reprojection
+// of the frame's other pixels
 int launch_march(mtc_ctx* c, const FrameConsts& fc, const OutputViews& ov, int full, hipStream_t s,
                  const KernelEvents& ke = KernelEvents(), bool with_reproj = false) {
-  if (with_reproj && c->desc.kernel != MTC_KERNEL_FAST && c->desc.kernel != MTC_KERNEL_PRECISION)
-    return mtc_fail(MTC_ERR_UNSUPPORTED, "march: no reprojection in this kernel's launch");
+  if (with_reproj && c->desc.kernel == MTC_KERNEL_PLAIN)
+    return mtc_fail(MTC_ERR_UNSUPPORTED, "march: no reprojection in the plain kernel's launch");
   VolumeViews vv = volume_views(c);
   hipError_t e;
   switch (c->desc.kernel) {
@@ -185,7 +180,7 @@ int launch_march(mtc_ctx* c, const FrameConsts& fc, const OutputViews& ov, int f
       int rc = learned_light_of(c, &ll);
       if (rc) return rc;
       e = launch_march_learned(fc, c->q_lf, c->q_hf, c->q_curl, c->q_weather, vv, ov, full, c->lanes_per_ray, ll,
-                               next_tile_order(c), s, ke);
+                               with_reproj, next_tile_order(c), s, ke);
       break;
     }
     default: return mtc_fail(MTC_ERR_UNSUPPORTED, "march: kernel variant not available in this build");
@@ -277,11 +272,6 @@ int mtc_create(const mtc_desc* d, mtc_ctx** out) {
   if (e == hipSuccess) e = hipMalloc((void**)&c->mask, px * sizeof(float4));
   if (e == hipSuccess) e = hipMalloc((void**)&c->counters, 16 * sizeof(unsigned long long));
   if (d->kernel != MTC_KERNEL_PLAIN) {
-    if (d->kernel == MTC_KERNEL_LEARNED) {
-      if (e == hipSuccess) e = hipStreamCreateWithFlags(&c->aux, hipStreamNonBlocking);
-      if (e == hipSuccess) e = hipEventCreateWithFlags(&c->fork, hipEventDisableTiming);
-      if (e == hipSuccess) e = hipEventCreateWithFlags(&c->join, hipEventDisableTiming);
-    }
     const size_t lf3 = (size_t)v.lf_dim * v.lf_dim * v.lf_dim, hf3 = (size_t)v.hf_dim * v.hf_dim * v.hf_dim;
     if (e == hipSuccess) e = hipMalloc((void**)&c->q_lf, 2 * lf3 * sizeof(uint4));  // 32-byte 2x2x2 LF entries
     if (e == hipSuccess) e = hipMalloc((void**)&c->q_hf, hf3 * sizeof(uint4));
@@ -386,12 +376,11 @@ int mtc_frame(mtc_ctx* c, const mtc_camera* cur, const mtc_camera* old, const mt
   ov.reproj_idx = nullptr;
   const int full = (c->desc.flags & MTC_FLAG_FULL_MARCH) ? 1 : 0;
   const bool timing = (c->desc.flags & MTC_FLAG_TIMING) != 0 && (c->frames_seen++ % (uint64_t)c->timing_stride) == 0;
-  // fast / precision 4x4 frames: one launch, the reprojection strips dealt
-  // after the march tiles (march_fast.hip)
-  const bool fused = !full && (c->desc.kernel == MTC_KERNEL_FAST || c->desc.kernel == MTC_KERNEL_PRECISION);
+  // optimised 4x4 frames: one launch, the reprojection strips dealt after
+  // the march tiles (march_fast.hip)
+  const bool fused = !full && c->desc.kernel != MTC_KERNEL_PLAIN;
   // timing: start / end events attached to each kernel's dispatch; the
-  // frame keeps its structure (the concurrent reprojection stays
-  // concurrent) and no marker sits between the kernels
+  // frame keeps its structure and no marker sits between the kernels
   std::array<hipEvent_t, 4> ev = {nullptr, nullptr, nullptr, nullptr};
   if (timing) {
     HIP_TRY(fold_timings(c, 255), "mtc_frame: timing events");
@@ -400,19 +389,10 @@ int mtc_frame(mtc_ctx* c, const mtc_camera* cur, const mtc_camera* old, const mt
       if (!ev[k]) return mtc_fail(MTC_ERR_HIP, "mtc_frame: hipEventCreate failed");
     }
   }
-  if (!full && !fused && c->aux) {
-    // learned light: reprojection of the other 15/16 of the pixels beside
-    // the march, on the second stream
-    HIP_TRY(hipEventRecord(c->fork, s), "mtc_frame: fork");
-    HIP_TRY(hipStreamWaitEvent(c->aux, c->fork, 0), "mtc_frame: fork");
-    HIP_TRY(launch_reproject(fc, ov, 2, c->aux, KernelEvents{ev[0], ev[1]}), "reprojection kernel launch");
-    HIP_TRY(hipEventRecord(c->join, c->aux), "mtc_frame: join");
-  } else if (!full && !fused) {
+  if (!full && !fused)
     HIP_TRY(launch_reproject(fc, ov, 0, s, KernelEvents{ev[0], ev[1]}), "reprojection kernel launch");
-  }
   rc = launch_march(c, fc, ov, full, s, KernelEvents{ev[2], ev[3]}, fused);
   if (rc) return rc;
-  if (!full && !fused && c->aux) HIP_TRY(hipStreamWaitEvent(s, c->join, 0), "mtc_frame: join");
   if (timing) c->pending.push_back(ev);
   c->latest = dst;
   return MTC_OK;

@@ -179,10 +179,6 @@ struct LearnedLight {
 struct mtc_ctx {
   mtc_desc desc;
   hipStream_t stream = nullptr;
-  // learned-light kernel: the reprojection runs on `aux` beside the march
-  // (fork / join events), the two writing disjoint pixels
-  hipStream_t aux = nullptr;
-  hipEvent_t fork = nullptr, join = nullptr;
   void* lf = nullptr;
   void* hf = nullptr;
   uint32_t* curl = nullptr;
