@@ -211,6 +211,50 @@ def run_batch(kernel: int, renders: int = 5) -> dict:
             "ms_per_batch": round(1000.0 * dt, 3), "mpixels_per_s": round(px / dt / 1e6, 1), "renders": renders}
 
 
+POSES_4K = [(yaw, elev) for yaw in (0.0, 90.0, 180.0, 270.0) for elev in (0.0, 20.0, 40.0, 60.0)]
+
+
+def run_4k(kernel: int, frames_per_pose: int = 4) -> dict:
+    """The high-resolution precision workload: 3840x2160, 192 primary steps
+    (preset 3), LF 256^3 and HF 64^3 FP16 volumes, 16 ground poses (yaw 0 /
+    90 / 180 / 270 x elevation 0 / 20 / 40 / 60 deg), each marched as
+    frames_per_pose consecutive 4x4 frames (0.5 deg/frame yaw) after a
+    warm-up pass over the first pose; the same frames with the plain kernel
+    for the ratio.  Timed between synchronisations."""
+    W4, H4 = 3840, 2160
+    params, sun = M.preset_params(3)
+    seqs = []
+    for yaw, elev in POSES_4K:
+        p, y = np.radians(elev), np.radians(yaw)
+        ref = (float(np.cos(p) * np.sin(y)), float(np.sin(p)), float(-np.cos(p) * np.cos(y)))
+        cam = M.Camera((0.0, 0.0, 0.0), ref, W4, H4, 45.0)
+        t = M.Time()
+        old, frames = cam.ubo(), []
+        for _ in range(frames_per_pose):
+            cam.rotate_about_up(YAW_DEG)
+            t.advance(DT)
+            frames.append((cam.ubo(), old, A.MtcTime.from_buffer_copy(t.ubo)))
+            old = frames[-1][0]
+        seqs.append(frames)
+    ms = {}
+    for name, k in (("plain", A.MTC_KERNEL_PLAIN), ("kernel", kernel)):
+        with M.CloudStage(W4, H4, M.VolumeSpec(256, 64, 128, 512, A.MTC_FMT_FP16), kernel=k) as st:
+            st.generate_volumes(1, 2)
+            for cur, old, tm in seqs[0]:
+                st.frame(cur, old, tm, sun, params)
+            st.synchronize()
+            t0 = time.perf_counter()
+            for frames in seqs:
+                for cur, old, tm in frames:
+                    st.frame(cur, old, tm, sun, params)
+            st.synchronize()
+            ms[name] = 1000.0 * (time.perf_counter() - t0) / (len(seqs) * frames_per_pose)
+    return {"workload": "3840x2160, 192 primary steps + 6 cone samples, LF 256^3 / HF 64^3 FP16 volumes, "
+                        "16 poses (yaw 0/90/180/270 x elevation 0/20/40/60 deg), 4x4 reprojection",
+            "frames": len(seqs) * frames_per_pose, "ms_per_frame": round(ms["kernel"], 4),
+            "plain_ms_per_frame": round(ms["plain"], 4), "speedup_vs_plain": round(ms["plain"] / ms["kernel"], 3)}
+
+
 def count_work(seq, warmup: int, steps: int) -> dict:
     st, _ = run_gpu(A.MTC_KERNEL_PLAIN, seq, warmup, 0, flags=A.MTC_FLAG_COUNTERS)
     st.reset_counters()
@@ -286,6 +330,7 @@ def main() -> int:
     ap.add_argument("--cpu-seconds", type=float, default=12.0)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-batch", action="store_true", help="skip the 96-view batch measurement")
+    ap.add_argument("--no-4k", action="store_true", help="skip the 4K / 192-step 16-pose measurement")
     args = ap.parse_args()
     if args.gpus != 1:
         print(json.dumps({"error": "the cloud stage runs on one GPU; only --gpus 1 is supported",
@@ -385,6 +430,9 @@ def main() -> int:
     if not args.no_batch and best in ("fast", "precision"):
         out["batch_96_views"] = run_batch(KERNELS[best])
         print(f"[bench] 96-view batch: {out['batch_96_views']}", file=sys.stderr, flush=True)
+    if not args.no_4k and best in ("fast", "precision"):
+        out["precision_4k_16_poses"] = run_4k(KERNELS[best])
+        print(f"[bench] 4K / 192 steps, 16 poses: {out['precision_4k_16_poses']}", file=sys.stderr, flush=True)
     if not args.no_cpu_baseline:
         out["cpu_baseline"] = cpu_baseline(seq, args.cpu_seconds)
     print(json.dumps(out))
