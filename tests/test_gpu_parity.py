@@ -237,6 +237,37 @@ def test_reprojection_of_the_fast_path_is_bit_identical(yaw, pitch):
     assert np.array_equal(outs[1][~sel], ref[~sel])
 
 
+@pytest.mark.parametrize("kernel", [A.MTC_KERNEL_FAST, A.MTC_KERNEL_PRECISION, A.MTC_KERNEL_LEARNED])
+def test_reprojection_strips_of_a_ragged_frame_are_bit_identical(kernel):
+    """The optimised launches carry the reprojection as 64 x 4 pixel strips
+    after the march tiles; on a frame whose width and height are not
+    multiples of the strip (203 x 75) every pixel the march does not
+    overwrite equals the FP32 oracle's bit for bit."""
+    W, H = 203, 75
+    params, sun = M.preset_params(2)
+    camera = ground_camera(W, H, 20.0)
+    old = camera.ubo()
+    camera.rotate_about_up(0.5)
+    camera.rotate_about_right(0.4)
+    cur = camera.ubo()
+    tm = _time(9).ubo
+    hist = np.random.default_rng(22).uniform(0.0, 4.0, size=(H, W, 4)).astype(np.float32)
+    with M.LightNet(capacity=1 << 10) as net:
+        net.init(3)
+        st, _ = _stage(W, H, dims=SMALL, vol=volumes(*SMALL), kernel=kernel)
+        with st:
+            if kernel == A.MTC_KERNEL_LEARNED:
+                net.attach(st)
+            st.set_history(hist)
+            st.frame(cur, old, tm, sun, params)
+            out = st.read_output()[0]
+    sel = np.zeros((H, W), bool)
+    xy = O.select_pixels(W, H, 9)
+    sel[xy[:, 1], xy[:, 0]] = True
+    ref, _, _ = O.frame(volumes(*SMALL), cur, old, tm, sun, params, W, H, prev=hist, precision=32)
+    assert np.array_equal(out[~sel].view(np.uint32), ref[~sel].view(np.uint32))
+
+
 @pytest.mark.parametrize("kernel", [A.MTC_KERNEL_PLAIN, A.MTC_KERNEL_PRECISION])
 def test_timing_events_sample_frames_without_changing_results(kernel):
     """MTC_FLAG_TIMING attaches events to one frame in `stride`; the frames
