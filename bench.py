@@ -211,6 +211,41 @@ def run_batch(kernel: int, renders: int = 5) -> dict:
             "ms_per_batch": round(1000.0 * dt, 3), "mpixels_per_s": round(px / dt / 1e6, 1), "renders": renders}
 
 
+# Algorithmic HBM bytes per pixel of the post passes: god rays = mask read +
+# footprint write (decode) + footprint, image read + image write (100 taps
+# along the sun line hit the footprint in cache); tone map = RGBA32F read +
+# RGBA8 write; TXAA = RGBA8 frame read + RGBA8 history read + RGBA8 write.
+POST_BYTES_PER_PX = {"god_rays": 80, "tone_map": 20, "txaa": 12}
+
+
+def run_post(kernel: int, seq, warmup: int, reps: int = 100) -> dict:
+    """The post passes at the workload's size after the warm-up frames of the
+    sequence: god rays, tone map (RGBA8), TXAA, each issued reps times back
+    to back between synchronisations (wall time per pass, so a pass's launch
+    gap is included)."""
+    st, _ = run_gpu(kernel, seq, warmup, 0)
+    params, sun = M.preset_params(PRESET)
+    cur, old, tm = seq[warmup - 1]
+    passes = (("god_rays", lambda: st.god_rays(cur, sun)), ("tone_map", lambda: st.tone_map(tm)),
+              ("txaa", lambda: st.txaa(cur, old, tm)))
+    out = {}
+    try:
+        for name, run in passes:
+            run()
+            st.synchronize()
+            t0 = time.perf_counter()
+            for _ in range(reps):
+                run()
+            st.synchronize()
+            us = 1e6 * (time.perf_counter() - t0) / reps
+            gbs = POST_BYTES_PER_PX[name] * W * H / (us * 1e-6) / 1e9
+            out[name] = {"us_per_pass": round(us, 2), "bytes_per_px": POST_BYTES_PER_PX[name],
+                         "gb_per_s": round(gbs, 1), "frac_of_8tbs": round(gbs / 8000.0, 4)}
+    finally:
+        st.close()
+    return out
+
+
 POSES_4K = [(yaw, elev) for yaw in (0.0, 90.0, 180.0, 270.0) for elev in (0.0, 20.0, 40.0, 60.0)]
 
 
@@ -331,6 +366,7 @@ def main() -> int:
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-batch", action="store_true", help="skip the 96-view batch measurement")
     ap.add_argument("--no-4k", action="store_true", help="skip the 4K / 192-step 16-pose measurement")
+    ap.add_argument("--no-post", action="store_true", help="skip the post-pass (god rays, tone map, TXAA) timing")
     args = ap.parse_args()
     if args.gpus != 1:
         print(json.dumps({"error": "the cloud stage runs on one GPU; only --gpus 1 is supported",
@@ -430,6 +466,9 @@ def main() -> int:
     if not args.no_batch and best in ("fast", "precision"):
         out["batch_96_views"] = run_batch(KERNELS[best])
         print(f"[bench] 96-view batch: {out['batch_96_views']}", file=sys.stderr, flush=True)
+    if not args.no_post:
+        out["post_passes_1080p"] = run_post(KERNELS[best], seq, args.warmup)
+        print(f"[bench] post passes: {out['post_passes_1080p']}", file=sys.stderr, flush=True)
     if not args.no_4k and best in ("fast", "precision"):
         out["precision_4k_16_poses"] = run_4k(KERNELS[best])
         print(f"[bench] 4K / 192 steps, 16 poses: {out['precision_4k_16_poses']}", file=sys.stderr, flush=True)

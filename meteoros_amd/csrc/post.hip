@@ -140,7 +140,10 @@ __global__ __launch_bounds__(256) void tone_map_kernel(const float4* __restrict_
   for (int k = 0; k < 3; ++k) {
     float c = uncharted2(2.5f * rgb[k]);
     c = c * whitemap;
-    o[k] = powf(c, 1.0f / 2.2f) + noise;
+    // GLSL defines pow(x, y) as exp2(y * log2(x)) (its precision is
+    // inherited from those two): v_log_f32 / v_exp_f32, not the library
+    // powf, which is ~100 instructions and made this pass VALU-bound
+    o[k] = __builtin_amdgcn_exp2f((1.0f / 2.2f) * __builtin_amdgcn_logf(c)) + noise;
   }
   const size_t i = (size_t)py * W + px;
   if (F32OUT) {
@@ -168,7 +171,9 @@ __device__ __forceinline__ float4 f4add(float4 a, float4 b) {
   return make_float4(a.x + b.x, a.y + b.y, a.z + b.z, a.w + b.w);
 }
 __device__ __forceinline__ float4 f4scale(float4 a, float s) { return make_float4(a.x * s, a.y * s, a.z * s, a.w * s); }
-__device__ __forceinline__ float4 f4div(float4 a, float s) { return make_float4(a.x / s, a.y / s, a.z / s, a.w / s); }
+__device__ __forceinline__ float4 f4div(float4 a, float s) {
+  return make_float4(a.x / s, a.y / s, a.z / s, a.w / s);
+}
 
 // imageLoad of the current RGBA8 frame; outside the image it returns 0
 __device__ __forceinline__ float4 load_cur(const uint32_t* __restrict__ img, int W, int H, int x, int y) {
