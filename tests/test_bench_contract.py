@@ -1,0 +1,45 @@
+"""The bench's command-line contract, on the CPU: more than one GPU and
+invalid step counts are refused with one JSON line and exit status 2 before
+any device work, and the FLOP weights of the roofline are the documented
+ones."""
+import json
+import os
+import subprocess
+import sys
+
+import pytest
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+
+
+def _bench(*args):
+    return subprocess.run([sys.executable, os.path.join(ROOT, "bench.py"), *args], capture_output=True, text=True,
+                          timeout=120, cwd=ROOT)
+
+
+@pytest.mark.parametrize("n", ["2", "8"])
+def test_more_than_one_gpu_is_refused(n):
+    r = _bench("--gpus", n)
+    assert r.returncode == 2
+    line = json.loads(r.stdout.strip().splitlines()[-1])
+    assert line["n_gpus"] == int(n) and "one GPU" in line["error"]
+
+
+@pytest.mark.parametrize("args", [("--steps", "0"), ("--warmup", "-1")])
+def test_invalid_step_counts_are_refused(args):
+    r = _bench(*args)
+    assert r.returncode == 2
+    assert "error" in json.loads(r.stdout.strip().splitlines()[-1])
+
+
+def test_march_flops_weights_match_the_documented_table():
+    sys.path.insert(0, ROOT)
+    import bench
+
+    assert bench.FLOPS["ray_marched"] == 500
+    assert bench.FLOPS["primary_step"] == 289
+    assert bench.FLOPS["cloud_step"] == 211
+    assert bench.FLOPS["cone_sample"] == 222
+    one_of_each = {"rays_marched": 1, "rays_ocean": 0, "rays_sky": 0, "primary_steps": 1, "cloud_steps": 1,
+                   "cone_samples": 6, "cone_erosions": 0}
+    assert bench.march_flops(one_of_each) == 500 + 289 + 211 + 6 * 222
