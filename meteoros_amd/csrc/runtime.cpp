@@ -375,6 +375,11 @@ int mtc_frame(mtc_ctx* c, const mtc_camera* cur, const mtc_camera* old, const mt
   ov.counters = (c->desc.flags & MTC_FLAG_COUNTERS) ? c->counters : nullptr;
   ov.reproj_idx = nullptr;
   const int full = (c->desc.flags & MTC_FLAG_FULL_MARCH) ? 1 : 0;
+  if (c->desc.kernel == MTC_KERNEL_LEARNED) {  // fail before any side effect
+    LearnedLight ll;
+    rc = learned_light_of(c, &ll);
+    if (rc) return rc;
+  }
   const bool timing = (c->desc.flags & MTC_FLAG_TIMING) != 0 && (c->frames_seen++ % (uint64_t)c->timing_stride) == 0;
   // optimised 4x4 frames: one launch, the reprojection strips dealt after
   // the march tiles (march_fast.hip)
@@ -382,17 +387,33 @@ int mtc_frame(mtc_ctx* c, const mtc_camera* cur, const mtc_camera* old, const mt
   // timing: start / end events attached to each kernel's dispatch; the
   // frame keeps its structure and no marker sits between the kernels
   std::array<hipEvent_t, 4> ev = {nullptr, nullptr, nullptr, nullptr};
+  // events taken for a frame that then fails go back to the pool
+  auto give_back = [&]() {
+    for (hipEvent_t& x : ev)
+      if (x) c->event_pool.push_back(x), x = nullptr;
+  };
   if (timing) {
     HIP_TRY(fold_timings(c, 255), "mtc_frame: timing events");
     for (int k = (full || fused) ? 2 : 0; k < 4; ++k) {
       ev[k] = take_event(c);
-      if (!ev[k]) return mtc_fail(MTC_ERR_HIP, "mtc_frame: hipEventCreate failed");
+      if (!ev[k]) {
+        give_back();
+        return mtc_fail(MTC_ERR_HIP, "mtc_frame: hipEventCreate failed");
+      }
     }
   }
-  if (!full && !fused)
-    HIP_TRY(launch_reproject(fc, ov, 0, s, KernelEvents{ev[0], ev[1]}), "reprojection kernel launch");
+  if (!full && !fused) {
+    const hipError_t e = launch_reproject(fc, ov, 0, s, KernelEvents{ev[0], ev[1]});
+    if (e != hipSuccess) {
+      give_back();
+      return mtc_fail(MTC_ERR_HIP, std::string("reprojection kernel launch: ") + hipGetErrorName(e));
+    }
+  }
   rc = launch_march(c, fc, ov, full, s, KernelEvents{ev[2], ev[3]}, fused);
-  if (rc) return rc;
+  if (rc) {
+    give_back();
+    return rc;
+  }
   if (timing) c->pending.push_back(ev);
   c->latest = dst;
   return MTC_OK;
