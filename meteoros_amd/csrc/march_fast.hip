@@ -730,13 +730,14 @@ __device__ __forceinline__ void march_tile(const FrameConsts& fc, const FastView
       if (COUNT && j == 0) n_steps += (unsigned)__popc(gm);
       gdone = ((gm >> (G - 1)) & 1u) == 0u;
     } else if (!gdone) {
-      const int step_flags = (pr.valid ? 1 : 0) | (pr.cloud ? 2 : 0);
+      // the group's valid / in-cloud bits from two wave ballots (no shuffle)
+      const unsigned long long vball = __ballot(pr.valid), cball = __ballot(pr.cloud);
+      const unsigned int vbits = (unsigned int)(vball >> gbase), cbits = (unsigned int)(cball >> gbase);
       bool last_valid = false, exited = false;
 #pragma unroll
       for (int i = 0; i < G; ++i) {
         const int src = gbase + i;
-        const int fl = G == 1 ? step_flags : __shfl(step_flags, src, 64);
-        const bool v = (fl & 1) != 0, cl = (fl & 2) != 0;
+        const bool v = ((vbits >> i) & 1u) != 0u, cl = ((cbits >> i) & 1u) != 0u;
         const float da = G == 1 ? pr.da : __shfl(pr.da, src, 64);
         const float e = G == 1 ? E : __shfl(E, src, 64);
         const int ch = COUNT ? (G == 1 ? cone_hits : __shfl(cone_hits, src, 64)) : 0;
