@@ -43,3 +43,28 @@ def test_march_flops_weights_match_the_documented_table():
     one_of_each = {"rays_marched": 1, "rays_ocean": 0, "rays_sky": 0, "primary_steps": 1, "cloud_steps": 1,
                    "cone_samples": 6, "cone_erosions": 0}
     assert bench.march_flops(one_of_each) == 500 + 289 + 211 + 6 * 222
+
+
+@pytest.mark.gpu
+def test_bench_line_carries_the_required_fields(capsys, monkeypatch):
+    """A short bench run prints one JSON line with the metric, the roofline of
+    the march launch and the CPU baseline, and its headline kernel meets the
+    PSNR bar.  Run in this process (no child process is started from one
+    that holds the GPU)."""
+    sys.path.insert(0, ROOT)
+    import bench
+
+    monkeypatch.setattr(sys, "argv", ["bench.py", "--steps", "4", "--warmup", "2", "--light-steps", "200",
+                                      "--cpu-seconds", "1", "--no-batch", "--no-4k", "--no-post"])
+    assert bench.main() == 0
+    line = json.loads(capsys.readouterr().out.strip().splitlines()[-1])
+    for key in ("metric", "value", "unit", "n_gpus", "steps", "warmup", "ms_per_step", "higher_is_better", "scaling",
+                "vs_baseline", "dtype", "data", "config", "roofline", "cpu_baseline"):
+        assert key in line, key
+    assert line["n_gpus"] == 1 and line["steps"] == 4 and line["value"] > 0.0
+    roof = line["roofline"]
+    assert roof["bound"] in ("hbm", "mfma") and roof["unit"] in ("GB/s", "TFLOP/s")
+    assert roof["achieved"] > 0.0 and abs(roof["frac"] - roof["achieved"] / roof["peak"]) < 1e-3
+    cpu = line["cpu_baseline"]
+    assert cpu["value"] > 0.0 and cpu["cores"] >= 1 and cpu["kind"] in ("port", "reference")
+    assert line["accuracy"]["psnr_db_fp64"] >= 45.0
