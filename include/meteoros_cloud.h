@@ -286,10 +286,13 @@ int mtc_generate_volumes(mtc_ctx* ctx, const mtc_noise_desc* noise);
 int mtc_get_volumes(mtc_ctx* ctx, void* lf, void* hf, void* curl, void* weather);
 
 /* One frame of the cloud compute stage (Renderer.cpp:653-722):
- * reprojection of the previous result into the current image, then the
- * ray march of the 1/16 pixels selected by time->frame_count_mod16, then
- * the ping-pong swap.  Enqueued on `stream` (hipStream_t, NULL = the
- * context's stream); uniforms are captured by value at enqueue time. */
+ * reprojection of the previous result into the current image and the ray
+ * march of the 1/16 pixels selected by time->frame_count_mod16, then the
+ * ping-pong swap.  No pixel is written twice: plain contexts run the two
+ * kernels in that order; the optimised contexts run one launch whose
+ * reprojection leaves the marched pixels out.  Enqueued on `stream`
+ * (hipStream_t, NULL = the context's stream); uniforms are captured by
+ * value at enqueue time. */
 int mtc_frame(mtc_ctx* ctx, const mtc_camera* cur, const mtc_camera* old,
               const mtc_time* time, const mtc_sun* sun,
               const mtc_cloud_params* params, void* stream);
