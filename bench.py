@@ -50,11 +50,13 @@ FLOPS = {
 }
 
 
-def hbm_traffic(kernel_name: str):
-    """HBM bytes per march launch of `kernel_name` from the committed PMC
-    summary (profiles/march_pmc_1080p.json: FETCH_SIZE and WRITE_SIZE of
-    rocprofv3 --pmc runs, each scaled by what the counter reads for a copy
-    kernel of known size), or None when the summary has no such entry."""
+def traffic_beyond_l2(kernel_name: str):
+    """Bytes per march launch of `kernel_name` that cross between the L2 and
+    the memory fabric (MALL hits included, so an upper bound on HBM bytes),
+    from the committed counter summary profiles/march_pmc_1080p.json:
+    FETCH_SIZE and WRITE_SIZE of rocprofv3 --pmc runs of this bench, each
+    scaled by what the counter reads for a copy kernel of known size.  Not
+    measured in this run; None when the summary has no such entry."""
     path = os.path.join(ROOT, "profiles", "march_pmc_1080p.json")
     try:
         with open(path) as f:
@@ -62,6 +64,17 @@ def hbm_traffic(kernel_name: str):
         return float(k["fetch_bytes"] + k["write_bytes"])
     except (OSError, KeyError, ValueError):
         return None
+
+
+def algorithmic_bytes(c: dict, frames: int) -> float:
+    """Unique HBM bytes one frame of the stage must move, per launch: the
+    previous image read once (16 B/px) and the reprojected pixels written
+    (16 B each), the marched pixels' colour and mask (32 B each), and the
+    four resources read once (LF 128^3 x 4 B, HF 32^3 x 4 B, curl 128^2 x
+    4 B, weather 512^2 x 4 B)."""
+    marched = (c["rays_marched"] + c["rays_ocean"] + c["rays_sky"]) / frames
+    volumes = 128 ** 3 * 4 + 32 ** 3 * 4 + 128 ** 2 * 4 + 512 ** 2 * 4
+    return W * H * 16 + (W * H - marched) * 16 + marched * 32 + volumes
 
 
 def march_flops(c: dict) -> float:
@@ -191,10 +204,14 @@ def batch_views(seed: int = 11):
     return cams, times, suns, params
 
 
-def run_batch(kernel: int, renders: int = 5) -> dict:
+def run_batch(kernel: int, renders: int = 5, check_rays: int = 0) -> dict:
     """Autotunes and times the 96-view batch (one persistent launch in a
-    hipGraph per render, uniforms uploaded each render)."""
+    hipGraph per render, uniforms uploaded each render).  With check_rays >
+    0 the CPU leg then checks every view of the last render against the FP64
+    oracle on a strided subset of about check_rays pixels and reports the
+    minimum and median PSNR per size class."""
     cams, times, suns, params = batch_views()
+    views = None
     with M.CloudStage(16, 16, M.VolumeSpec(128, 32, 128, 512, A.MTC_FMT_UNORM8), kernel=kernel) as st:
         st.generate_volumes(1, 2)
         with M.Batch(st, BATCH_SIZES) as b:
@@ -206,9 +223,32 @@ def run_batch(kernel: int, renders: int = 5) -> dict:
                 b.render(cams, times, suns, params)
             st.synchronize()
             dt = (time.perf_counter() - t0) / renders
+            if check_rays > 0:
+                views = [b.read(v)[0] for v in range(len(BATCH_SIZES))]
+                vols = st.get_volumes()
     px = sum(w * h for w, h in BATCH_SIZES)
-    return {"views": len(BATCH_SIZES), "pixels": px, "tuning": best, "autotune_best_ms": round(tune_ms, 3),
-            "ms_per_batch": round(1000.0 * dt, 3), "mpixels_per_s": round(px / dt / 1e6, 1), "renders": renders}
+    out = {"views": len(BATCH_SIZES), "pixels": px, "tuning": best, "autotune_best_ms": round(tune_ms, 3),
+           "ms_per_batch": round(1000.0 * dt, 3), "mpixels_per_s": round(px / dt / 1e6, 1), "renders": renders}
+    if views is not None:
+        from oracle import oracle as O
+
+        vol = O.Volumes(*vols, 128, 32, 128, 512, A.MTC_FMT_UNORM8)
+        per_class = {}
+        for v, (w, h) in enumerate(BATCH_SIZES):
+            s = max(1, int(np.sqrt(w * h / check_rays)))
+            xy = np.stack(np.meshgrid(np.arange(v % s, w, s), np.arange(v % s, h, s)), -1).reshape(-1, 2)
+            xy = xy.astype(np.int32)
+            ref, _, _, _ = O.march_pixels(vol, cams[v], times[v], suns[v], params[v], w, h, xy, 64)
+            keep = O.outside_sun_disk(cams[v], times[v], params[v], xy, w, h)
+            e = O.error_vs_fp64(views[v][xy[:, 1], xy[:, 0], :3], ref[:, :3], keep)
+            per_class.setdefault(f"{w}x{h}", []).append(e)
+        out["psnr_vs_fp64"] = {
+            cls: {"views": len(es), "min_db": round(min(e["psnr_db"] for e in es), 2),
+                  "median_db": round(float(np.median([e["psnr_db"] for e in es])), 2),
+                  "max_err_frac_of_peak": float(max(e["max_err_frac_of_peak"] for e in es)),
+                  "rays_per_view": int(np.median([e["rays"] for e in es]))}
+            for cls, es in per_class.items()}
+    return out
 
 
 # Algorithmic HBM bytes per pixel of the post passes: god rays = mask read +
@@ -249,7 +289,7 @@ def run_post(kernel: int, seq, warmup: int, reps: int = 100) -> dict:
 POSES_4K = [(yaw, elev) for yaw in (0.0, 90.0, 180.0, 270.0) for elev in (0.0, 20.0, 40.0, 60.0)]
 
 
-def run_4k(kernel: int, frames_per_pose: int = 4) -> dict:
+def run_4k(kernel: int, frames_per_pose: int = 4, check_rays: int = 0) -> dict:
     """The high-resolution precision workload: 3840x2160, 192 primary steps
     (preset 3), LF 256^3 and HF 64^3 FP16 volumes, 16 ground poses (yaw 0 /
     90 / 180 / 270 x elevation 0 / 20 / 40 / 60 deg), each marched as
@@ -271,7 +311,7 @@ def run_4k(kernel: int, frames_per_pose: int = 4) -> dict:
             frames.append((cam.ubo(), old, A.MtcTime.from_buffer_copy(t.ubo)))
             old = frames[-1][0]
         seqs.append(frames)
-    ms = {}
+    ms, errs = {}, []
     for name, k in (("plain", A.MTC_KERNEL_PLAIN), ("kernel", kernel)):
         with M.CloudStage(W4, H4, M.VolumeSpec(256, 64, 128, 512, A.MTC_FMT_FP16), kernel=k) as st:
             st.generate_volumes(1, 2)
@@ -284,10 +324,34 @@ def run_4k(kernel: int, frames_per_pose: int = 4) -> dict:
                     st.frame(cur, old, tm, sun, params)
             st.synchronize()
             ms[name] = 1000.0 * (time.perf_counter() - t0) / (len(seqs) * frames_per_pose)
-    return {"workload": "3840x2160, 192 primary steps + 6 cone samples, LF 256^3 / HF 64^3 FP16 volumes, "
-                        "16 poses (yaw 0/90/180/270 x elevation 0/20/40/60 deg), 4x4 reprojection",
-            "frames": len(seqs) * frames_per_pose, "ms_per_frame": round(ms["kernel"], 4),
-            "plain_ms_per_frame": round(ms["plain"], 4), "speedup_vs_plain": round(ms["plain"] / ms["kernel"], 3)}
+            if name == "kernel" and check_rays > 0:
+                # checker (after the timed region): the last frame of every
+                # pose, re-marched alone, against the FP64 oracle on
+                # check_rays of its marched rays
+                from oracle import oracle as O
+
+                vol = O.Volumes(*st.get_volumes(), 256, 64, 128, 512, A.MTC_FMT_FP16)
+                rng = np.random.default_rng(3)
+                for frames in seqs:
+                    cur, old, tm = frames[-1]
+                    st.frame(cur, old, tm, sun, params)
+                    rgba, _ = st.read_output()
+                    xy = O.select_pixels(W4, H4, tm.frame_count_mod16)
+                    xy = xy[rng.choice(xy.shape[0], check_rays, replace=False)]
+                    ref, _, _, _ = O.march_pixels(vol, cur, tm, sun, params, W4, H4, xy, 64)
+                    keep = O.outside_sun_disk(cur, tm, params, xy, W4, H4)
+                    errs.append(O.error_vs_fp64(rgba[xy[:, 1], xy[:, 0], :3], ref[:, :3], keep))
+    out = {"workload": "3840x2160, 192 primary steps + 6 cone samples, LF 256^3 / HF 64^3 FP16 volumes, "
+                       "16 poses (yaw 0/90/180/270 x elevation 0/20/40/60 deg), 4x4 reprojection",
+           "frames": len(seqs) * frames_per_pose, "ms_per_frame": round(ms["kernel"], 4),
+           "plain_ms_per_frame": round(ms["plain"], 4), "speedup_vs_plain": round(ms["plain"] / ms["kernel"], 3)}
+    if errs:
+        out["error_vs_fp64"] = {"rays_per_pose": check_rays,
+                                "psnr_db_min": round(min(e["psnr_db"] for e in errs), 2),
+                                "psnr_db_median": round(float(np.median([e["psnr_db"] for e in errs])), 2),
+                                "max_err_frac_of_peak": float(max(e["max_err_frac_of_peak"] for e in errs)),
+                                "p999_err_frac_of_peak_max": float(max(e["p999_err_frac_of_peak"] for e in errs))}
+    return out
 
 
 def count_work(seq, warmup: int, steps: int) -> dict:
@@ -313,17 +377,33 @@ def accuracy(rgba, vols, seq, warmup: int, steps: int, subsample: int = 4) -> di
     vol = O.Volumes(*vols, 128, 32, 128, 512, A.MTC_FMT_UNORM8)
     xy = O.select_pixels(W, H, tm.frame_count_mod16)[::subsample]
     got = rgba[xy[:, 1], xy[:, 0], :3].astype(np.float64)
-    r64, _, _, _ = O.march_pixels(vol, cur, tm, sun, params, W, H, xy, 64)
-    r32, _, _, _ = O.march_pixels(vol, cur, tm, sun, params, W, H, xy, 32)
+    r64, _, s64, _ = O.march_pixels(vol, cur, tm, sun, params, W, H, xy, 64)
+    r32, _, s32, _ = O.march_pixels(vol, cur, tm, sun, params, W, H, xy, 32)
     keep = O.outside_sun_disk(cur, tm, params, xy, W, H)
     everything = np.ones(len(xy), bool)
-    ref32 = r32[:, :3]
-    peak32 = float(np.abs(ref32).max())
-    rel = np.abs(got - ref32) / np.maximum(np.abs(ref32), 1e-3 * peak32)
-    return {"psnr_db_fp64": round(O.psnr_vs_fp64(got, r64[:, :3], keep), 2),
-            "psnr_db_fp64_with_sun_disk": round(O.psnr_vs_fp64(got, r64[:, :3], everything), 2),
+    ref32, ref64 = r32[:, :3], r64[:, :3]
+    peak32, peak64 = float(np.abs(ref32).max()), float(np.abs(ref64).max())
+    rel = (np.abs(got - ref32) / np.maximum(np.abs(ref32), 1e-3 * peak32)).max(axis=1)
+    rel64 = (np.abs(got - ref64) / np.maximum(np.abs(ref64), 1e-3 * peak64)).max(axis=1)
+    rel_32_64 = (np.abs(ref32 - ref64) / np.maximum(np.abs(ref64), 1e-3 * peak64)).max(axis=1)
+    w = int(np.argmax(rel))
+    e = O.error_vs_fp64(got, ref64, keep)
+    e_all = O.error_vs_fp64(got, ref64, everything)
+    return {"psnr_db_fp64": round(O.psnr_vs_fp64(got, ref64, keep), 2),
+            "psnr_db_fp64_with_sun_disk": round(O.psnr_vs_fp64(got, ref64, everything), 2),
+            "max_err_frac_of_peak_fp64": e["max_err_frac_of_peak"],
+            "p999_err_frac_of_peak_fp64": e["p999_err_frac_of_peak"],
+            "max_err_frac_of_peak_fp64_with_sun_disk": e_all["max_err_frac_of_peak"],
             "sun_disk_rays_excluded": int((~keep).sum()), "rays_checked": int(xy.shape[0]),
-            "max_rel_vs_fp32_oracle": float(rel.max())}
+            "max_rel_vs_fp32_oracle": float(rel.max()),
+            # the ray behind max_rel_vs_fp32_oracle: its error against the FP64
+            # oracle, the FP32 oracle's own error there, and both oracles' step
+            # counts (they differ where the FP32 shell quadratic moves t_k)
+            "worst_vs_fp32_oracle": {"x": int(xy[w, 0]), "y": int(xy[w, 1]), "rel_vs_fp32": float(rel[w]),
+                                     "rel_vs_fp64": float(rel64[w]), "fp32_oracle_rel_vs_fp64": float(rel_32_64[w]),
+                                     "steps_fp32": int(s32[w]), "steps_fp64": int(s64[w]),
+                                     "in_sun_disk": bool(not keep[w])},
+            "rays_with_fp32_fp64_step_difference": int((s32 != s64).sum())}
 
 
 def cpu_baseline(seq, budget_s: float) -> dict:
@@ -408,7 +488,12 @@ def main() -> int:
         valid = [n for n in names if n in ("fast", "precision")]
     else:
         valid = [n for n in names if results[n]["acc"]["psnr_db_fp64"] >= PSNR_BAR_DB]
-    best = min(valid or names, key=lambda n: results[n]["seconds"])
+    if not valid:
+        print(json.dumps({"error": "no kernel reaches the PSNR bar against the FP64 oracle",
+                          "bar_db": PSNR_BAR_DB,
+                          "psnr_db": {n: results[n]["acc"]["psnr_db_fp64"] for n in names if results[n]["acc"]}}))
+        return 1
+    best = min(valid, key=lambda n: results[n]["seconds"])
     r = results[best]
     dt = r["seconds"]
     ms = 1000.0 * dt / args.steps
@@ -418,8 +503,15 @@ def main() -> int:
     reproj_ms = kt["reproject_ms"] / max(kt["frames"], 1)
     flops_per_launch = march_flops(work) / args.steps
     achieved = flops_per_launch / (march_ms * 1e-3) / 1e12
-    roof = {"bound": "mfma", "pipe": "valu-fp32", "kernel": "march", "achieved": round(achieved, 3),
-            "peak": 157.3, "unit": "TFLOP/s", "frac": round(achieved / 157.3, 4), "traffic": hbm_traffic(best),
+    traffic = traffic_beyond_l2(best)
+    alg_bytes = algorithmic_bytes(work, args.steps)
+    roof = {"bound": "valu", "pipe": "valu-fp32", "kernel": "march", "achieved": round(achieved, 3),
+            "peak": 157.3, "unit": "TFLOP/s", "frac": round(achieved / 157.3, 4), "traffic": traffic,
+            "traffic_is": "bytes between L2 and the memory fabric per launch (TCC FETCH_SIZE + WRITE_SIZE, "
+                          "MALL hits included), from the committed counter summary profiles/march_pmc_1080p.json, "
+                          "not measured in this run",
+            "algorithmic_bytes": alg_bytes,
+            "traffic_over_algorithmic": round(traffic / alg_bytes, 2) if traffic else None,
             "flops_per_launch": flops_per_launch, "avg_launch_ms": round(march_ms, 4)}
     if best in ("fast", "precision"):
         roof["launch"] = "march tiles + the frame's reprojection strips in one launch (its duration covers both)"
@@ -464,13 +556,13 @@ def main() -> int:
         "accuracy": r["acc"],
     }
     if not args.no_batch and best in ("fast", "precision"):
-        out["batch_96_views"] = run_batch(KERNELS[best])
+        out["batch_96_views"] = run_batch(KERNELS[best], check_rays=0 if args.no_cpu_baseline else 500)
         print(f"[bench] 96-view batch: {out['batch_96_views']}", file=sys.stderr, flush=True)
     if not args.no_post:
         out["post_passes_1080p"] = run_post(KERNELS[best], seq, args.warmup)
         print(f"[bench] post passes: {out['post_passes_1080p']}", file=sys.stderr, flush=True)
     if not args.no_4k and best in ("fast", "precision"):
-        out["precision_4k_16_poses"] = run_4k(KERNELS[best])
+        out["precision_4k_16_poses"] = run_4k(KERNELS[best], check_rays=0 if args.no_cpu_baseline else 1500)
         print(f"[bench] 4K / 192 steps, 16 poses: {out['precision_4k_16_poses']}", file=sys.stderr, flush=True)
     if not args.no_cpu_baseline:
         out["cpu_baseline"] = cpu_baseline(seq, args.cpu_seconds)

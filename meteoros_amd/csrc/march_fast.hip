@@ -781,6 +781,11 @@ __device__ __forceinline__ void march_tile(const FrameConsts& fc, const FastView
       float a = accum * smoothstepf(0.0f, 1.0f, fminf(1.0f, remap(ray.d.y, fc.fade_start, fc.fade_end, 0.0f, 1.0f)));
       color = make_float4(mixf(bg.x, col, a), mixf(bg.y, col, a), mixf(bg.z, col, a), 1.0f);
       mask = encode_float_rgba(25.0f * fminf(0.05f, 1.0f - god));
+      // :775-779, reachable only when cloud_fade_start < 0.05
+      if (ray.d.y < 0.05f) {
+        const float s = fmaxf(5.0f, ray.d.y);
+        mask = make_float4(mask.x * s, mask.y * s, mask.z * s, mask.w * s);
+      }
     }
     if (j == 0) {
       const size_t o = (size_t)py * (size_t)W + (size_t)px;

@@ -430,8 +430,10 @@ template <class T> static inline T height_gradient(T h, T type) {
 }
 // sampleLowFrequency, :499-540
 template <class T>
-static inline T sample_low(const Frame<T>& f, const Volumes& vol, v3<T> p, v3<T> unskewed, T h) {
+static inline T sample_low(const Frame<T>& f, const Volumes& vol, v3<T> p, v3<T> unskewed, T h,
+                           C4<T>* lf_out = nullptr) {
   C4<T> n = sample3d<T>(vol.lf, vol.format, vol.lf_dim, p);
+  if (lf_out) *lf_out = n;
   T fbm = (n.g * T(0.625)) + (n.b * T(0.25)) + (n.a * T(0.125));
   fbm = clampT(fbm, T(0), T(1));
   T base = remapClamped(n.r, (fbm - T(0.9)), T(1.0), T(0), T(1));
@@ -447,7 +449,8 @@ static inline T sample_low(const Frame<T>& f, const Volumes& vol, v3<T> p, v3<T>
 }
 // erodeCloudWithHighFrequency, :542-563
 template <class T>
-static inline T erode(const Frame<T>& f, const Volumes& vol, T base, v3<T> p, T h) {
+static inline T erode(const Frame<T>& f, const Volumes& vol, T base, v3<T> p, T h, C4<T>* curl_out = nullptr,
+                      C4<T>* hf_out = nullptr, v3<T>* p_out = nullptr) {
   (void)f;
   C4<T> cn = sample2d<T>(vol.curl, vol.curl_dim, p.x, p.y);
   // point.xy += curlNoise.xy * (1.0 - height_fraction) * 0.5
@@ -455,6 +458,9 @@ static inline T erode(const Frame<T>& f, const Volumes& vol, T base, v3<T> p, T 
   p.x = p.x + (cn.r * k) * T(0.5);
   p.y = p.y + (cn.g * k) * T(0.5);
   C4<T> hn = sample3d<T>(vol.hf, vol.format, vol.hf_dim, p);
+  if (curl_out) *curl_out = cn;
+  if (hf_out) *hf_out = hn;
+  if (p_out) *p_out = p;
   T fbm = (hn.r * T(0.625)) + (hn.g * T(0.25)) + (hn.b * T(0.125));
   T mod = clampT(mixT(fbm, T(1) - fbm, clampT(h * T(2), T(0), T(1))), T(0), T(1));
   return remap(base, mod * T(0.005), T(1.0), T(0), T(1));
@@ -472,11 +478,32 @@ struct LightRecord {
   double w[2];     // weather (coverage, type) at the cone's first sample
 };
 
+// The values of one executed primary step, recorded when ray_march is given
+// a trace list, so that tests can pin each stage of the step against closed
+// forms.  All doubles (k and in_cloud too) so the record is one flat array.
+// No GLSL counterpart.
+struct StepTrace {
+  double k, in_cloud, t;
+  double pos[3];   // ray.o + t * (ray.d + jitter)
+  double sp[3];    // getRelativePositionInAtmosphere(pos) / sample_divisor
+  double sk[3];    // skewSamplePointWithWind(sp, h)
+  double h;        // getRelativeHeightInAtmosphere
+  double lf[4];    // the primary LF sample (texture(LF, sk))
+  double base;     // sampleLowFrequency(...) * base_density_factor
+  double curl[2];  // curl sample of the primary erosion (in-cloud steps)
+  double hf[3];    // HF sample of the primary erosion
+  double ero[3];   // the point the HF volume is sampled at
+  double hfd, accum, dal, energy, transmittance;  // after the step
+  double cone_lf_r[6];   // r of the LF sample of cone sample i
+  double cone_pos[6][3]; // the point cone sample i reads the LF at
+  double cone_e[6];      // cone sample i's eroded density (0 when empty)
+};
+
 // rayMarch, :565-688
 template <class T>
 static v3<T> ray_march(const Frame<T>& f, const Volumes& vol, const Ray<T>& ray, v3<T> c, v3<T> startPos,
                        T start_t, T end_t, int pixelID, T& accum, Counters& cnt,
-                       std::vector<LightRecord>* rec = nullptr) {
+                       std::vector<LightRecord>* rec = nullptr, std::vector<StepTrace>* trace = nullptr) {
   T d = dot(ray.d, V(T(0), T(1), T(0)));
   T maxSteps;
   if (f.step_mode == MTC_STEPS_FIXED) maxSteps = (T)f.steps_max;
@@ -520,10 +547,20 @@ static v3<T> ray_march(const Frame<T>& f, const Volumes& vol, const Ray<T>& ray,
     sp = sp / f.sample_divisor;
     T h = rel_height(f, pos, c, startPos, ray.d, ray.o);
     v3<T> sk = skew(f, sp, h);
-    T base = sample_low(f, vol, sk, sp, h) * f.base_density_factor;
+    StepTrace st{};
+    C4<T> lfs{}, cls{}, hfs{};
+    v3<T> ep{};
+    T base = sample_low(f, vol, sk, sp, h, &lfs) * f.base_density_factor;
+    if (trace) {
+      st.k = iter; st.t = (double)t; st.h = (double)h; st.base = (double)base;
+      const v3<T> vs[3] = {pos, sp, sk};
+      double* ds[3] = {st.pos, st.sp, st.sk};
+      for (int j = 0; j < 3; ++j) { ds[j][0] = (double)vs[j].x; ds[j][1] = (double)vs[j].y; ds[j][2] = (double)vs[j].z; }
+      st.lf[0] = (double)lfs.r; st.lf[1] = (double)lfs.g; st.lf[2] = (double)lfs.b; st.lf[3] = (double)lfs.a;
+    }
     if (base > T(0)) {
       cnt.cloud_steps++;
-      T hfd = erode(f, vol, base * f.erosion_scale, sk, h);
+      T hfd = erode(f, vol, base * f.erosion_scale, sk, h, &cls, &hfs, &ep);
       accum += hfd * f.accum_scale;
       T dal = T(0);
       T terms[6] = {T(0), T(0), T(0), T(0), T(0), T(0)};
@@ -531,7 +568,12 @@ static v3<T> ray_march(const Frame<T>& f, const Volumes& vol, const Ray<T>& ray,
         cnt.cone_samples++;
         v3<T> lp = pos + ((stepSize * kern[i]) * (T)i);
         v3<T> slp = rel_pos(f, lp, c);
-        T cb = sample_low(f, vol, slp, slp, h);
+        C4<T> cls_i{};
+        T cb = sample_low(f, vol, slp, slp, h, &cls_i);
+        if (trace && i < 6) {
+          st.cone_lf_r[i] = (double)cls_i.r;
+          st.cone_pos[i][0] = (double)slp.x; st.cone_pos[i][1] = (double)slp.y; st.cone_pos[i][2] = (double)slp.z;
+        }
         if (cb > T(0)) {
           cnt.cone_erosions++;
           T e = erode(f, vol, f.cone_erosion_scale * cb, sk, h);
@@ -568,6 +610,19 @@ static v3<T> ray_march(const Frame<T>& f, const Volumes& vol, const Ray<T>& ray,
       T E = light_energy(h, dal, base, HG_light, cos_angle, stepSize, f.brightness);
       transmittance = mixT(transmittance, E, T(1) - accum);
       ret = ret + V(transmittance, transmittance, transmittance);
+      if (trace) {
+        st.in_cloud = 1.0;
+        st.curl[0] = (double)cls.r; st.curl[1] = (double)cls.g;
+        st.hf[0] = (double)hfs.r; st.hf[1] = (double)hfs.g; st.hf[2] = (double)hfs.b;
+        st.ero[0] = (double)ep.x; st.ero[1] = (double)ep.y; st.ero[2] = (double)ep.z;
+        st.hfd = (double)hfd; st.dal = (double)dal; st.energy = (double)E;
+        for (int i = 0; i < 6; ++i) st.cone_e[i] = (double)terms[i];
+      }
+    }
+    if (trace) {
+      st.accum = (double)accum;
+      st.transmittance = (double)transmittance;
+      trace->push_back(st);
     }
     if (accum >= T(1)) {
       accum = T(1);
@@ -581,7 +636,8 @@ static v3<T> ray_march(const Frame<T>& f, const Volumes& vol, const Ray<T>& ray,
 // main of cloudRayMarch.comp, :690-826, for one chosen pixel.
 template <class T>
 static void march_pixel(const Frame<T>& f, const Volumes& vol, int px, int py, int pixelID, T rgba[4], T mask[4],
-                        Counters& cnt, std::vector<LightRecord>* rec = nullptr) {
+                        Counters& cnt, std::vector<LightRecord>* rec = nullptr,
+                        std::vector<StepTrace>* trace = nullptr) {
   T ux = (T)px / (T)f.W, uy = (T)py / (T)f.H;
   uy = T(1) - uy;
   v3<T> eye = V(-f.eye_ubo[0], -f.eye_ubo[1], -f.eye_ubo[2]);
@@ -613,7 +669,7 @@ static void march_pixel(const Frame<T>& f, const Volumes& vol, int px, int py, i
   Isect<T> in = ray_sphere(ray.o, ray.d, c, f.r_in);
   Isect<T> out = ray_sphere(ray.o, ray.d, c, f.r_out);
   T accum = T(0);
-  v3<T> col = ray_march(f, vol, ray, c, in.point, in.t, out.t, pixelID, accum, cnt, rec);
+  v3<T> col = ray_march(f, vol, ray, c, in.point, in.t, out.t, pixelID, accum, cnt, rec, trace);
   T godAccum = accum;
   accum = accum * smoothstepT(T(0), T(1), std::fmin(T(1), remap(ray.d.y, f.fade_start, f.fade_end, T(0), T(1))));
   rgba[0] = mixT(bg.x, col.x, accum);
@@ -622,6 +678,10 @@ static void march_pixel(const Frame<T>& f, const Volumes& vol, int px, int py, i
   rgba[3] = T(1);
   T grey = T(25.0) * std::fmin(T(0.05), T(1) - godAccum);
   encode_float_rgba(grey, mask);
+  // :775-779, reachable only when cloud_fade_start < 0.05
+  if (ray.d.y < T(0.05)) {
+    for (int i = 0; i < 4; ++i) mask[i] = mask[i] * std::fmax(T(5.0), ray.d.y);
+  }
 }
 
 // main of reprojection.comp, :193-244, for one pixel.  idx (optional)

@@ -13,7 +13,10 @@ import subprocess
 import numpy as np
 
 _HERE = os.path.dirname(os.path.abspath(__file__))
-LIB_PATH = os.path.join(_HERE, "_build", "liboracle.so")
+# METEOROS_ORACLE_LIB points the tests at another build of this oracle (the
+# mutation tests build deliberately broken copies on the CPU and check that
+# the pins reject them).
+LIB_PATH = os.environ.get("METEOROS_ORACLE_LIB") or os.path.join(_HERE, "_build", "liboracle.so")
 
 # The boundary structs are shared with the product's C header; import the
 # ctypes mirrors from there (they are plain type definitions).
@@ -42,6 +45,10 @@ def lib() -> C.CDLL:
         h.orc_light_samples.argtypes = [vp, vp, vp, vp, P(A.MtcVolumeDesc), P(A.MtcCamera), P(A.MtcTime),
                                         P(A.MtcSun), P(A.MtcCloudParams), C.c_int, C.c_int, vp, C.c_int64, C.c_int,
                                         C.c_int64, vp, vp, vp, vp, vp]
+        h.orc_march_trace.restype = C.c_int64
+        h.orc_march_trace.argtypes = [vp, vp, vp, vp, P(A.MtcVolumeDesc), P(A.MtcCamera), P(A.MtcTime), P(A.MtcSun),
+                                      P(A.MtcCloudParams), C.c_int, C.c_int, C.c_int, C.c_int, C.c_int, C.c_int64,
+                                      vp, vp, vp]
         h.orc_god_rays.restype = C.c_int
         h.orc_god_rays.argtypes = [P(A.MtcCamera), P(A.MtcSun), C.c_int, C.c_int, vp, vp, vp, C.c_int, vp]
         h.orc_tone_map.restype = C.c_int
@@ -184,6 +191,31 @@ def light_samples(vol: Volumes, cam, time, sun, params, W, H, xy: np.ndarray, pr
             out = (pix[:m], step[:m], x[:m], y[:m])
             return out + (ex[:m],) if extra else out
         cap = int(m)
+
+
+STEP_TRACE = np.dtype([("k", "f8"), ("in_cloud", "f8"), ("t", "f8"), ("pos", "f8", 3), ("sp", "f8", 3),
+                       ("sk", "f8", 3), ("h", "f8"), ("lf", "f8", 4), ("base", "f8"), ("curl", "f8", 2),
+                       ("hf", "f8", 3), ("ero", "f8", 3), ("hfd", "f8"), ("accum", "f8"), ("dal", "f8"),
+                       ("energy", "f8"), ("transmittance", "f8"), ("cone_lf_r", "f8", 6), ("cone_pos", "f8", (6, 3)),
+                       ("cone_e", "f8", 6)])
+assert STEP_TRACE.itemsize == 61 * 8
+
+
+def march_trace(vol: Volumes, cam, time, sun, params, W, H, px, py, precision=64):
+    """Every executed primary step of pixel (px, py)'s march as a STEP_TRACE
+    record array (position, sample point, height, wind-skewed point, the LF /
+    curl / HF samples, density, accumulation, cone sample points and terms,
+    light energy, transmittance), plus the pixel's colour and mask."""
+    cap = 4096
+    buf = np.zeros(cap * 61, dtype=np.float64)
+    rgba = np.zeros(4, dtype=np.float64)
+    mask = np.zeros(4, dtype=np.float64)
+    n = lib().orc_march_trace(*vol.ptrs(), C.byref(vol.desc), C.byref(cam), C.byref(time), C.byref(sun),
+                              C.byref(params), W, H, int(px), int(py), precision, cap, buf.ctypes.data,
+                              rgba.ctypes.data, mask.ctypes.data)
+    if n < 0 or n > cap:
+        raise ValueError("orc_march_trace rejected its arguments")
+    return buf[: n * 61].view(STEP_TRACE).copy(), rgba, mask
 
 
 def god_rays(cam, sun, rgba: np.ndarray, mask: np.ndarray, precision=32):
@@ -371,6 +403,18 @@ def outside_sun_disk(cam, time, params, xy, W, H, radius_deg=SUN_DISK_RADIUS_DEG
     s = np.array(params.sky_sun_location[:3], dtype=np.float64)
     s /= np.linalg.norm(s)
     return d @ s < np.cos(np.radians(radius_deg))
+
+
+def error_vs_fp64(got_rgb, ref_rgb, keep):
+    """Per-pixel error of the kept pixels against the FP64 reference: PSNR
+    (as psnr_vs_fp64), and the largest and the 99.9th-percentile absolute
+    RGB error, each as a fraction of the peak (max of the FP64 reference)."""
+    g = np.asarray(got_rgb, dtype=np.float64)[keep]
+    r = np.asarray(ref_rgb, dtype=np.float64)[keep]
+    peak = float(r.max())
+    err = np.abs(g - r).max(axis=1) / peak
+    return {"psnr_db": psnr_vs_fp64(got_rgb, ref_rgb, keep), "max_err_frac_of_peak": float(err.max()),
+            "p999_err_frac_of_peak": float(np.quantile(err, 0.999)), "rays": int(r.shape[0])}
 
 
 def psnr_vs_fp64(got_rgb, ref_rgb, keep):

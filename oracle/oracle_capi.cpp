@@ -130,6 +130,40 @@ int64_t orc_light_samples(const void* lf, const void* hf, const uint8_t* curl, c
   return total;
 }
 
+// Every executed primary step of one pixel's march (main of
+// cloudRayMarch.comp with pixelID = frame_count_mod16), as orc::StepTrace
+// records (61 doubles each): writes up to `cap` records into `out`, the
+// pixel's colour and mask into rgba/mask (4 doubles each, may be NULL), and
+// returns the number of steps (0 for ocean and sky-only pixels), or -1 on
+// bad arguments.
+int64_t orc_march_trace(const void* lf, const void* hf, const uint8_t* curl, const uint8_t* weather,
+                        const mtc_volume_desc* vd, const mtc_camera* cam, const mtc_time* tm, const mtc_sun* sun,
+                        const mtc_cloud_params* p, int W, int H, int px, int py, int precision, int64_t cap,
+                        double* out, double* rgba, double* mask) {
+  static_assert(sizeof(orc::StepTrace) == 61 * sizeof(double), "StepTrace is 61 doubles");
+  if (!lf || !hf || !curl || !weather || !vd || !cam || !tm || !sun || !p || W <= 0 || H <= 0 || cap < 0 ||
+      (cap > 0 && !out))
+    return -1;
+  Volumes vol = make_volumes(lf, hf, curl, weather, vd);
+  const int pid = tm->frame_count_mod16;
+  std::vector<orc::StepTrace> tr;
+  Counters cnt;
+  double c4[4], m4[4];
+  if (precision == 64) {
+    Frame<double> f = orc::make_frame<double>(*cam, *tm, *sun, *p, W, H);
+    orc::march_pixel<double>(f, vol, px, py, pid, c4, m4, cnt, nullptr, &tr);
+  } else {
+    Frame<float> f = orc::make_frame<float>(*cam, *tm, *sun, *p, W, H);
+    float cf[4], mf[4];
+    orc::march_pixel<float>(f, vol, px, py, pid, cf, mf, cnt, nullptr, &tr);
+    for (int i = 0; i < 4; ++i) { c4[i] = cf[i]; m4[i] = mf[i]; }
+  }
+  for (size_t i = 0; i < tr.size() && (int64_t)i < cap; ++i) std::memcpy(out + 61 * i, &tr[i], sizeof(orc::StepTrace));
+  if (rgba) for (int i = 0; i < 4; ++i) rgba[i] = c4[i];
+  if (mask) for (int i = 0; i < 4; ++i) mask[i] = m4[i];
+  return (int64_t)tr.size();
+}
+
 // The 4x4 selection of cloudRayMarch.comp:695-704 for one frame: writes the
 // list of chosen pixels (x, y) into xy (capacity ceil(W/4)*ceil(H/4)) and
 // returns their number.

@@ -63,8 +63,12 @@ def test_bench_line_carries_the_required_fields(capsys, monkeypatch):
         assert key in line, key
     assert line["n_gpus"] == 1 and line["steps"] == 4 and line["value"] > 0.0
     roof = line["roofline"]
-    assert roof["bound"] in ("hbm", "mfma") and roof["unit"] in ("GB/s", "TFLOP/s")
+    assert roof["bound"] == "valu" and roof["pipe"] == "valu-fp32" and roof["unit"] == "TFLOP/s"
+    assert roof["algorithmic_bytes"] > 0.0 and "traffic_is" in roof
     assert roof["achieved"] > 0.0 and abs(roof["frac"] - roof["achieved"] / roof["peak"]) < 1e-3
     cpu = line["cpu_baseline"]
     assert cpu["value"] > 0.0 and cpu["cores"] >= 1 and cpu["kind"] in ("port", "reference")
-    assert line["accuracy"]["psnr_db_fp64"] >= 45.0
+    acc = line["accuracy"]
+    assert acc["psnr_db_fp64"] >= 45.0
+    assert 0.0 <= acc["p999_err_frac_of_peak_fp64"] <= acc["max_err_frac_of_peak_fp64"]
+    assert {"x", "y", "rel_vs_fp64", "steps_fp32", "steps_fp64"} <= set(acc["worst_vs_fp32_oracle"])

@@ -102,3 +102,58 @@ def test_batch_rejects_bad_input():
             with pytest.raises(M.MtcError) as e:
                 b.render([cam], [t.ubo], [s], [p])  # no volumes yet
             assert e.value.code == A.MTC_ERR_STATE
+
+
+def _stride(w, h, target=2000):
+    return max(1, int(np.sqrt(w * h / target)))
+
+
+@pytest.mark.parametrize("kernel", [A.MTC_KERNEL_FAST, A.MTC_KERNEL_PRECISION])
+def test_96_view_batch_every_view_vs_fp64(kernel):
+    """The 96-view workload exactly as the bench renders it (bench.BATCH_SIZES
+    and bench.batch_views(): 6 cube faces 512^2 at 90 deg, 40 probes 128^2,
+    40 views 256x144, 10 views 1920x1080; seeded poses and suns, 64-160
+    steps), autotuned: every view reaches 45 dB against the FP64 oracle on a
+    strided subset of its pixels (about 2000 rays), and the first view of
+    each size class equals its single-view mtc_march_full bit for bit."""
+    import bench
+
+    sizes = bench.BATCH_SIZES
+    cams, times, suns, params = bench.batch_views()
+    with M.CloudStage(16, 16, M.VolumeSpec(*DIMS), kernel=kernel) as st:
+        st.generate_volumes(1, 2)
+        vols = st.get_volumes()
+        vol = O.Volumes(*vols, *DIMS)
+        with M.Batch(st, sizes) as b:
+            best, ms = b.autotune(cams, times, suns, params)
+            print(f"kernel {kernel}: autotuned {best}, {ms:.3f} ms per 96-view render")
+            b.render(cams, times, suns, params)
+            st.synchronize()
+            views = [b.read(v) for v in range(len(sizes))]
+    bits = lambda x: np.ascontiguousarray(x).view(np.uint32)  # noqa: E731
+    firsts = {}
+    for v, s in enumerate(sizes):
+        firsts.setdefault(s, v)
+    for (w, h), v in firsts.items():
+        with M.CloudStage(w, h, M.VolumeSpec(*DIMS), kernel=kernel) as one:
+            one.set_volumes(*vols)
+            one.march_full(cams[v], times[v], suns[v], params[v])
+            rgba, mask = one.read_output()
+        assert np.array_equal(bits(rgba), bits(views[v][0])), f"view {v} ({w}x{h}) colour differs from its single-view march"
+        assert np.array_equal(bits(mask), bits(views[v][1])), f"view {v} ({w}x{h}) mask differs from its single-view march"
+    per_class, below = {}, []
+    for v, (w, h) in enumerate(sizes):
+        s = _stride(w, h)
+        xy = np.stack(np.meshgrid(np.arange(v % s, w, s), np.arange(v % s, h, s)), -1).reshape(-1, 2).astype(np.int32)
+        ref, _, _, _ = O.march_pixels(vol, cams[v], times[v], suns[v], params[v], w, h, xy, 64)
+        got = views[v][0][xy[:, 1], xy[:, 0], :3]
+        keep = O.outside_sun_disk(cams[v], times[v], params[v], xy, w, h)
+        e = O.error_vs_fp64(got, ref[:, :3], keep)
+        per_class.setdefault(f"{w}x{h}", []).append(e)
+        if not e["psnr_db"] >= 45.0:
+            below.append((v, w, h, e))
+    for cls, es in per_class.items():
+        ps = np.array([e["psnr_db"] for e in es])
+        print(f"kernel {kernel} {cls}: {len(es)} views, PSNR min {ps.min():.2f} median {np.median(ps):.2f} dB, "
+              f"max err {max(e['max_err_frac_of_peak'] for e in es):.2e} of peak")
+    assert not below, f"views below 45 dB: {below}"

@@ -303,3 +303,37 @@ def test_timing_events_sample_frames_without_changing_results(kernel):
                 with pytest.raises(M.MtcError):
                     st.set_timing_stride(0)
     assert np.array_equal(outs[0].view(np.uint32), outs[1].view(np.uint32))
+
+
+@pytest.mark.parametrize("kernel", [A.MTC_KERNEL_PLAIN, A.MTC_KERNEL_FAST, A.MTC_KERNEL_PRECISION])
+def test_low_fade_start_scales_the_horizon_mask(kernel):
+    """With cloud_fade_start = 0.01 rays with 0.01 <= dir.y < 0.05 are
+    marched, and their god-ray mask is multiplied by max(5, dir.y)
+    (cloudRayMarch.comp:775-779, unreachable with the shipped 0.06).  The
+    plain kernel matches the FP32 oracle to 1e-3; the optimised kernels'
+    decoded masks match the FP64 oracle's on average to 5 x 2e-3."""
+    W, H = 160, 90
+    params, sun = M.preset_params(1)
+    params.cloud_fade_start = 0.01
+    cam = ground_camera(W, H, 2.0).ubo()
+    tm = _time(2).ubo
+    st, v = _stage(W, H, kernel=kernel)
+    with st:
+        st.march_full(cam, tm, sun, params)
+        rgba, mask = st.read_output()
+    xy = np.stack(np.meshgrid(np.arange(W), np.arange(H)), -1).reshape(-1, 2).astype(np.int32)
+    dy = O.ray_directions(cam, tm, xy, W, H)[:, 1].reshape(H, W)
+    low = (dy >= 0.011) & (dy < 0.049)
+    assert low.sum() > 500
+    if kernel == A.MTC_KERNEL_PLAIN:
+        ref, refm, _ = O.frame(v, cam, None, tm, sun, params, W, H, precision=32, full=True)
+        assert not within_rel(rgba, ref).any()
+        assert not within_rel(mask, refm, floor_frac=1.0).any()
+        assert np.abs(refm[low]).max() > 1.0  # a scaled encoding: components above 1
+    else:
+        r64, m64, _, _ = O.march_pixels(v, cam, tm, sun, params, W, H, xy, 64)
+        dec = lambda m: m @ (1.0 / np.array([1.0, 255.0, 65025.0, 16581375.0]))  # noqa: E731
+        got = dec(mask.reshape(-1, 4).astype(np.float64))
+        ref = dec(m64)
+        assert np.abs(got - ref)[low.reshape(-1)].mean() < 1e-2
+        assert np.abs(m64[low.reshape(-1)]).max() > 1.0
