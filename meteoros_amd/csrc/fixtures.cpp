@@ -49,6 +49,12 @@ struct Image8 {
   std::vector<uint8_t> rgba;  // top row first
 };
 
+// Largest decoded image: 2^26 pixels (8192^2, 256 MiB of RGBA8), the largest
+// map mtc_create accepts.  Sizes come from file headers, so they are checked
+// against this bound and against what the file's bytes can hold before any
+// buffer is allocated.
+constexpr size_t kMaxPixels = size_t(1) << 26;
+
 // ---------------------------------------------------------------------------
 // TGA
 // ---------------------------------------------------------------------------
@@ -64,7 +70,13 @@ int decode_tga(const std::vector<uint8_t>& d, Image8& img, std::string& err) {
     return err = "TGA depth " + std::to_string(bpp) + " not supported", MTC_ERR_UNSUPPORTED;
   const int bytes = bpp / 8;
   size_t pos = 18 + (size_t)idlen;
-  std::vector<uint8_t> px((size_t)w * h * bytes);
+  const size_t npx = (size_t)w * (size_t)h;
+  if (npx > kMaxPixels) return err = "image larger than 2^26 pixels", MTC_ERR_INVALID_ARG;
+  // raw data must be in the file; an RLE packet (1 + bytes) covers at most 128 pixels
+  const size_t avail = d.size() > pos ? d.size() - pos : 0;
+  if (!rle && npx * bytes > avail) return err = "truncated pixel data", MTC_ERR_INVALID_ARG;
+  if (rle && npx > 128 * avail) return err = "truncated RLE data", MTC_ERR_INVALID_ARG;
+  std::vector<uint8_t> px(npx * bytes);
   size_t o = 0;
   while (o < px.size()) {
     if (!rle) {
@@ -153,6 +165,7 @@ int decode_png(const std::vector<uint8_t>& d, Image8& img, std::string& err) {
     pos += 12 + (size_t)len;
   }
   if (w <= 0 || h <= 0 || w > (1 << 16) || h > (1 << 16)) return err = "bad PNG size", MTC_ERR_INVALID_ARG;
+  if ((size_t)w * (size_t)h > kMaxPixels) return err = "image larger than 2^26 pixels", MTC_ERR_INVALID_ARG;
   if (interlace) return err = "interlaced PNG not supported", MTC_ERR_UNSUPPORTED;
   int channels;
   switch (ctype) {
@@ -168,6 +181,8 @@ int decode_png(const std::vector<uint8_t>& d, Image8& img, std::string& err) {
   if (ctype == 3 && palette.empty()) return err = "palette image without PLTE", MTC_ERR_INVALID_ARG;
   const int bpp = channels * depth / 8;  // bytes per pixel
   const size_t stride = (size_t)w * bpp;
+  // deflate expands at most 1032:1, so a short IDAT cannot hold a large image
+  if ((stride + 1) * (size_t)h > 1032 * idat.size() + 64) return err = "truncated IDAT stream", MTC_ERR_INVALID_ARG;
   std::vector<uint8_t> raw((stride + 1) * (size_t)h);
   uLongf rawlen = (uLongf)raw.size();
   if (uncompress(raw.data(), &rawlen, idat.data(), (uLong)idat.size()) != Z_OK || rawlen != raw.size())
@@ -231,7 +246,7 @@ int decode_any(const std::string& path, Image8& img, std::string& err) {
 
 extern "C" {
 
-int mtc_load_image_rgba8(const char* path, int32_t* width, int32_t* height, uint8_t* out, size_t n_bytes) {
+int mtc_load_image_rgba8(const char* path, int32_t* width, int32_t* height, uint8_t* out, size_t n_bytes) try {
   if (!path) return mtc_fail(MTC_ERR_INVALID_ARG, "mtc_load_image_rgba8: null path");
   Image8 img;
   std::string err;
@@ -244,9 +259,13 @@ int mtc_load_image_rgba8(const char* path, int32_t* width, int32_t* height, uint
     std::memcpy(out, img.rgba.data(), img.rgba.size());
   }
   return MTC_OK;
+} catch (const std::bad_alloc&) {
+  return mtc_fail(MTC_ERR_OUT_OF_MEMORY, "mtc_load_image_rgba8: out of host memory");
+} catch (...) {
+  return mtc_fail(MTC_ERR_INVALID_ARG, "mtc_load_image_rgba8: decoding failed");
 }
 
-int mtc_load_tga_slices(const char* dir, const char* base, int32_t n, uint8_t* out, size_t n_bytes) {
+int mtc_load_tga_slices(const char* dir, const char* base, int32_t n, uint8_t* out, size_t n_bytes) try {
   if (!dir || !base || !out) return mtc_fail(MTC_ERR_INVALID_ARG, "mtc_load_tga_slices: null pointer");
   if (n < 1 || n > 1024) return mtc_fail(MTC_ERR_INVALID_ARG, "mtc_load_tga_slices: n must be in 1..1024");
   const size_t slice = (size_t)n * n * 4;
@@ -261,6 +280,10 @@ int mtc_load_tga_slices(const char* dir, const char* base, int32_t n, uint8_t* o
     std::memcpy(out + slice * z, img.rgba.data(), slice);
   }
   return MTC_OK;
+} catch (const std::bad_alloc&) {
+  return mtc_fail(MTC_ERR_OUT_OF_MEMORY, "mtc_load_tga_slices: out of host memory");
+} catch (...) {
+  return mtc_fail(MTC_ERR_INVALID_ARG, "mtc_load_tga_slices: decoding failed");
 }
 
 }  // extern "C"

@@ -92,3 +92,39 @@ def test_shipped_textures_load_as_pillow_reads_them():
         a = arr.reshape(-1, 4).astype(np.float64) / 255.0
         assert np.allclose(a.mean(0), stats[name]["mean"], atol=1e-12), name
         assert np.allclose(a.std(0), stats[name]["std"], atol=1e-12), name
+
+
+def test_headers_claiming_huge_images_are_refused_before_allocating(tmp_path):
+    """Sizes come from the file header: a 65535^2 TGA of 18 bytes, an RLE TGA
+    whose few packets cannot cover its claimed size, and a 65536^2 PNG with a
+    tiny IDAT are refused with an error status (no allocation of the claimed
+    size, no exception across the C ABI)."""
+    import struct
+    import zlib
+
+    from meteoros_amd import _abi as A
+
+    def tga(w, h, type_, body=b""):
+        return bytes([0, 0, type_]) + b"\x00" * 9 + struct.pack("<HHBB", w, h, 32, 0x28) + body
+
+    cases = {
+        "raw_huge.tga": tga(65535, 65535, 2),
+        "raw_short.tga": tga(64, 64, 2, b"\x01" * 100),
+        "rle_huge.tga": tga(65535, 65535, 10, b"\xff" + b"\x00" * 4),
+    }
+
+    def chunk(kind, data):
+        return struct.pack(">I", len(data)) + kind + data + struct.pack(">I", zlib.crc32(kind + data))
+
+    ihdr = struct.pack(">IIBBBBB", 65536, 65536, 8, 6, 0, 0, 0)
+    cases["huge.png"] = (b"\x89PNG\r\n\x1a\n" + chunk(b"IHDR", ihdr) + chunk(b"IDAT", zlib.compress(b"\x00" * 64))
+                         + chunk(b"IEND", b""))
+    ihdr = struct.pack(">IIBBBBB", 16384, 16384, 8, 6, 0, 0, 0)
+    cases["too_many_pixels.png"] = (b"\x89PNG\r\n\x1a\n" + chunk(b"IHDR", ihdr)
+                                    + chunk(b"IDAT", zlib.compress(b"\x00" * 64)) + chunk(b"IEND", b""))
+    for name, data in cases.items():
+        p = tmp_path / name
+        p.write_bytes(data)
+        with pytest.raises(M.MtcError) as e:
+            M.load_image_rgba8(str(p))
+        assert e.value.code == A.MTC_ERR_INVALID_ARG, name
