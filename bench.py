@@ -35,18 +35,20 @@ YAW_DEG, PITCH_DEG, DT = 0.5, 20.0, 1.0 / 60.0
 METRIC = "ray-steps/s and ms/frame at 1080p 128 steps vs plain FP32 kernel at PSNR≥45 dB"
 
 # Algorithmic FP32 operations per unit of work of cloudRayMarch.comp /
-# reprojection.comp as written in the GLSL (add, sub, mul, div, sqrt and
-# each transcendental = 1; fma = 2; comparisons, min/max, floor and int
-# conversions = 0).  Derivation per function in DESIGN.md.
+# reprojection.comp as the GLSL states them (add, sub, mul, div, sqrt and
+# each transcendental = 1; fma = 2; comparisons, min/max, floor, abs and
+# conversions = 0), counted by running the oracle's march and reprojection
+# with an operation-counting scalar on this workload (oracle/flop_count.hpp;
+# tests/test_flop_weights.py holds this table to that count).
 FLOPS = {
-    "ray_marched": 500,     # castRay, Preetham sky, 2 shell hits, march set-up, composite, mask
-    "ray_ocean": 80,        # castRay + ocean mix
-    "ray_sky": 183,         # castRay + Preetham sky
-    "primary_step": 289,    # jitter, position, relative height, wind skew, LF + weather sample
-    "cloud_step": 211,      # erosion (curl + HF), accumulation, light energy, transmittance mix
-    "cone_sample": 222,     # cone position, LF + weather sample
-    "cone_erosion": 170,    # curl + HF erosion of a cone sample
-    "reproject_pixel": 270,  # castRay, inner hit, old-camera projection, 10 taps, average
+    "ray_marched": 587,     # castRay, Preetham sky, 2 shell hits, march set-up, composite, mask
+    "ray_ocean": 91,        # castRay + ocean mix
+    "ray_sky": 250,         # castRay + Preetham sky
+    "primary_step": 309,    # jitter, position, relative height, wind skew, LF + weather sample, density
+    "cloud_step": 210,      # erosion (curl + HF), accumulation, light energy, transmittance mix
+    "cone_sample": 235,     # cone position, LF + weather sample, density
+    "cone_erosion": 170,    # erosion of a cone sample (curl + HF at the step's point)
+    "reproject_pixel": 286,  # castRay, inner hit, old-camera projection, 10 taps, average
 }
 
 

@@ -41,6 +41,28 @@
 
 namespace orc {
 
+// <cmath> through these names (called unqualified), so that the counting
+// scalar of flop_count.hpp can overload them by argument-dependent lookup.
+template <class T> static inline T m_floor(T x) { return std::floor(x); }
+template <class T> static inline T m_sqrt(T x) { return std::sqrt(x); }
+template <class T> static inline T m_exp(T x) { return std::exp(x); }
+template <class T> static inline T m_pow(T x, T y) { return std::pow(x, y); }
+template <class T> static inline T m_acos(T x) { return std::acos(x); }
+template <class T> static inline T m_cos(T x) { return std::cos(x); }
+template <class T> static inline T m_fabs(T x) { return std::fabs(x); }
+template <class T> static inline T m_fma(T a, T b, T c) { return std::fma(a, b, c); }
+template <class T> static inline T m_fmax(T a, T b) { return std::fmax(a, b); }
+template <class T> static inline T m_fmin(T a, T b) { return std::fmin(a, b); }
+template <class T> static inline T m_nearbyint(T x) { return std::nearbyint(x); }
+
+// Work regions of the march, for the operation counter of flop_count.hpp
+// (a no-op for float and double): per-ray set-up and composite, primary
+// step, in-cloud step, cone sample, cone erosion, reprojected pixel.
+enum FlopRegion { FR_RAY, FR_PRIMARY, FR_CLOUD, FR_CONE, FR_CONE_EROSION, FR_REPROJ, FR_COUNT };
+template <class T> struct Region {
+  static void set(int) {}
+};
+
 template <class T> struct v3 {
   T x, y, z;
 };
@@ -54,7 +76,7 @@ template <class T> static inline v3<T> operator*(v3<T> a, v3<T> b) { return V(a.
 template <class T> static inline v3<T> operator/(v3<T> a, T s) { return V(a.x / s, a.y / s, a.z / s); }
 template <class T> static inline v3<T> operator/(v3<T> a, v3<T> b) { return V(a.x / b.x, a.y / b.y, a.z / b.z); }
 template <class T> static inline T dot(v3<T> a, v3<T> b) { return a.x * b.x + a.y * b.y + a.z * b.z; }
-template <class T> static inline T length(v3<T> a) { return std::sqrt(dot(a, a)); }
+template <class T> static inline T length(v3<T> a) { return m_sqrt(dot(a, a)); }
 // GLSL normalize(v) = v / length(v)
 template <class T> static inline v3<T> normalize(v3<T> a) {
   T l = length(a);
@@ -63,15 +85,15 @@ template <class T> static inline v3<T> normalize(v3<T> a) {
 template <class T> static inline v3<T> cross(v3<T> a, v3<T> b) {
   return V(a.y * b.z - b.y * a.z, a.z * b.x - b.z * a.x, a.x * b.y - b.x * a.y);
 }
-template <class T> static inline T clampT(T x, T lo, T hi) { return std::fmin(std::fmax(x, lo), hi); }
+template <class T> static inline T clampT(T x, T lo, T hi) { return m_fmin(m_fmax(x, lo), hi); }
 // GLSL mix(x, y, a) = x*(1-a) + y*a
 template <class T> static inline T mixT(T x, T y, T a) { return x * (T(1) - a) + y * a; }
-template <class T> static inline T fractT(T x) { return x - std::floor(x); }
+template <class T> static inline T fractT(T x) { return x - m_floor(x); }
 template <class T> static inline T smoothstepT(T e0, T e1, T x) {
   T t = clampT((x - e0) / (e1 - e0), T(0), T(1));
   return t * t * (T(3) - T(2) * t);
 }
-template <class T> static inline T fmaT(T a, T b, T c) { return std::fma(a, b, c); }
+template <class T> static inline T fmaT(T a, T b, T c) { return m_fma(a, b, c); }
 
 // cloudRayMarch.comp:153-169
 template <class T> static inline T remap(T v, T omin, T omax, T nmin, T nmax) {
@@ -129,7 +151,7 @@ template <class T> static inline C4<T> unorm_scale(C4<T> c, int fmt) {
 // texture(sampler3D, p): linear filter, REPEAT, LOD 0 (Texture3D.cpp:92-134)
 template <class T> static C4<T> sample3d(const void* base, int fmt, int N, v3<T> s) {
   T ux = s.x * (T)N - T(0.5), uy = s.y * (T)N - T(0.5), uz = s.z * (T)N - T(0.5);
-  T fx = std::floor(ux), fy = std::floor(uy), fz = std::floor(uz);
+  T fx = m_floor(ux), fy = m_floor(uy), fz = m_floor(uz);
   T ax = ux - fx, ay = uy - fy, az = uz - fz;
   long long ix = (long long)fx, iy = (long long)fy, iz = (long long)fz;
   const long long m = N - 1;
@@ -149,7 +171,7 @@ template <class T> static C4<T> sample3d(const void* base, int fmt, int N, v3<T>
 // texture(sampler2D, p): linear filter, REPEAT, UNORM8 (Image.cpp:305-346)
 template <class T> static C4<T> sample2d(const uint8_t* base, int N, T sx, T sy) {
   T ux = sx * (T)N - T(0.5), uy = sy * (T)N - T(0.5);
-  T fx = std::floor(ux), fy = std::floor(uy);
+  T fx = m_floor(ux), fy = m_floor(uy);
   T ax = ux - fx, ay = uy - fy;
   long long ix = (long long)fx, iy = (long long)fy;
   const long long m = N - 1;
@@ -296,8 +318,8 @@ template <class T> static Isect<T> ray_sphere(v3<T> rO, v3<T> rD, v3<T> c, T R) 
   T C = dot(rO, rO) - T(1);
   T disc = B * B - T(4) * A * C;
   if (disc < T(0)) return is;
-  T t = (-B - std::sqrt(disc)) / (T(2) * A);
-  if (t < T(0)) t = (-B + std::sqrt(disc)) / (T(2) * A);
+  T t = (-B - m_sqrt(disc)) / (T(2) * A);
+  if (t < T(0)) t = (-B + m_sqrt(disc)) / (T(2) * A);
   if (t >= T(0)) {
     v3<T> p = rO + t * rD;
     is.valid = true;
@@ -313,21 +335,21 @@ template <class T> static Isect<T> ray_sphere(v3<T> rO, v3<T> rD, v3<T> c, T R) 
 // HenyeyGreenstein / HGModified, :295-306
 template <class T> static inline T hg(T c, T g) {
   T num = T(1) - g * g;
-  T den = std::pow((T(1) + g * g - T(2) * g * c), T(1.5));
+  T den = m_pow((T(1) + g * g - T(2) * g * c), T(1.5));
   return (num / den) * (T)0.07957747154594767;
 }
 template <class T> static inline T hg_modified(T c, T g, T si, T ss) {
-  return std::fmax(hg(c, g), si * hg(c, T(0.99) - ss));
+  return m_fmax(hg(c, g), si * hg(c, T(0.99) - ss));
 }
 
 // GetLightEnergy, :331-388 (third instance, "MEDIUM")
 template <class T>
 static inline T light_energy(T h, T dl, T ds, T phase, T cosang, T /*step*/, T brightness) {
-  T prim = std::exp(-dl);
-  T sec = std::exp(-dl);
-  T att = std::fmax(remap(cosang, T(0.7), T(1.0), sec, sec * T(0.25)), prim);
-  T depth = T(0.05) + std::pow(ds, clampT(remap(h * T(0.125), T(0.3), T(0.85), T(0.5), T(2.0)), T(0.5), T(2.0)));
-  T vert = std::pow(clampT(remap(h * T(1.5), T(0.07), T(0.34), T(0.1), T(1.0)), T(0.1), T(1.0)), T(0.8));
+  T prim = m_exp(-dl);
+  T sec = m_exp(-dl);
+  T att = m_fmax(remap(cosang, T(0.7), T(1.0), sec, sec * T(0.25)), prim);
+  T depth = T(0.05) + m_pow(ds, clampT(remap(h * T(0.125), T(0.3), T(0.85), T(0.5), T(2.0)), T(0.5), T(2.0)));
+  T vert = m_pow(clampT(remap(h * T(1.5), T(0.07), T(0.34), T(0.1), T(1.0)), T(0.1), T(1.0)), T(0.8));
   T insc = depth * vert;
   return att * prim * insc * phase * brightness;
 }
@@ -339,7 +361,7 @@ template <class T> static inline v3<T> rayleigh_total() {
 }
 template <class T> static inline v3<T> calc_beta_r(const Frame<T>& f) {
   T rayleigh = T(2);
-  T sunFade = T(1) - clampT(T(1) - std::exp(f.sky_sun.y / T(450000.0)), T(0), T(1));
+  T sunFade = T(1) - clampT(T(1) - m_exp(f.sky_sun.y / T(450000.0)), T(0), T(1));
   return rayleigh_total<T>() * (rayleigh - T(1) + sunFade);
 }
 // MIE_CONST (:85): pi (2 pi / lambda)^(v - 2) K at lambda = 680/550/450 nm,
@@ -356,25 +378,25 @@ template <class T> static inline v3<T> calc_beta_v() {
 }
 template <class T> static inline T calc_sun_intensity(const Frame<T>& f) {
   T zc = clampT(normalize(f.sky_sun).y, T(-1), T(1));
-  return T(1000.0) * std::fmax(T(0), T(1) - std::pow((T)2.718281828459,
-                                                     -(((T)1.6110731557 - std::acos(zc)) / T(1.5))));
+  return T(1000.0) * m_fmax(T(0), T(1) - m_pow((T)2.718281828459,
+                                                     -(((T)1.6110731557 - m_acos(zc)) / T(1.5))));
 }
 // the optical length's "inverse" (:438): Kasten's relative optical air mass
 // 1 / (cos z + 0.15 (93.885 - z_deg)^-1.253)
 template <class T> static inline T relative_air_mass(T zenith) {
-  return T(1.0) / (std::cos(zenith) + T(0.15) * std::pow(T(93.885) - ((zenith * T(180.0)) / (T)3.14159265), T(-1.253)));
+  return T(1.0) / (m_cos(zenith) + T(0.15) * m_pow(T(93.885) - ((zenith * T(180.0)) / (T)3.14159265), T(-1.253)));
 }
 template <class T> static v3<T> atmosphere_color(const Frame<T>& f, v3<T> dir, v3<T> sunDir, T sunIntensity) {
   sunDir = normalize(sunDir);
   T sunE = sunIntensity * calc_sun_intensity(f);
   v3<T> BR = calc_beta_r(f);
   v3<T> BM = calc_beta_v<T>();
-  T zenith = std::acos(std::fmax(T(0), dir.y));
+  T zenith = m_acos(m_fmax(T(0), dir.y));
   T inv = relative_air_mass(zenith);
   T sR = T(8.4E3) * inv;
   T sM = T(1.25E3) * inv;
   v3<T> ex = (-BR) * sR + BM * sM;
-  v3<T> fex = V(std::exp(ex.x), std::exp(ex.y), std::exp(ex.z));
+  v3<T> fex = V(m_exp(ex.x), m_exp(ex.y), m_exp(ex.z));
   T cosTheta = dot(sunDir, dir);
   T rPhase = rayleigh_phase(cosTheta * T(0.5) + T(0.5));
   v3<T> bRT = BR * rPhase;
@@ -384,9 +406,9 @@ template <class T> static v3<T> atmosphere_color(const Frame<T>& f, v3<T> dir, v
   yDot = yDot * (yDot * yDot * yDot * yDot);
   v3<T> betas = (bRT + bMT) / (BR + BM);
   v3<T> a = (sunE * betas) * (V(T(1), T(1), T(1)) - fex);
-  v3<T> Lin = V(std::pow(a.x, T(1.5)), std::pow(a.y, T(1.5)), std::pow(a.z, T(1.5)));
+  v3<T> Lin = V(m_pow(a.x, T(1.5)), m_pow(a.y, T(1.5)), m_pow(a.z, T(1.5)));
   v3<T> b = (sunE * betas) * fex;
-  v3<T> pb = V(std::pow(b.x, T(0.5)), std::pow(b.y, T(0.5)), std::pow(b.z, T(0.5)));
+  v3<T> pb = V(m_pow(b.x, T(0.5)), m_pow(b.y, T(0.5)), m_pow(b.z, T(0.5)));
   T yc = clampT(yDot, T(0), T(1));
   v3<T> m = V(mixT(T(1), pb.x, yc), mixT(T(1), pb.y, yc), mixT(T(1), pb.z, yc));
   Lin = Lin * m;
@@ -404,7 +426,7 @@ static inline T rel_height(const Frame<T>& f, v3<T> point, v3<T> c, v3<T> start,
   T ls = length(start - eye);
   v3<T> n = normalize(point - c);
   T cosT = dot(dir, n);
-  T num = std::fabs(cosT * (lr - ls));
+  T num = m_fabs(cosT * (lr - ls));
   return num / f.thickness;
 }
 // getRelativePositionInAtmosphere, :188-191
@@ -420,9 +442,9 @@ template <class T> static inline v3<T> skew(const Frame<T>& f, v3<T> p, T h) {
 // getDensityHeightGradientForPoint, :475-487 (weather-coverage mode only)
 template <class T> static inline T height_gradient(T h, T type) {
   h = clampT(h, T(0), T(1));
-  T cu = std::fmax(T(0), remap(h, T(0.01), T(0.3), T(0), T(1)) * remap(h, T(0.6), T(0.95), T(1), T(0)));
-  T sc = std::fmax(T(0), remap(h, T(0), T(0.25), T(0), T(1)) * remap(h, T(0.3), T(0.65), T(1), T(0)));
-  T st = std::fmax(T(0), remap(h, T(0), T(0.1), T(0), T(1)) * remap(h, T(0.2), T(0.3), T(1), T(0)));
+  T cu = m_fmax(T(0), remap(h, T(0.01), T(0.3), T(0), T(1)) * remap(h, T(0.6), T(0.95), T(1), T(0)));
+  T sc = m_fmax(T(0), remap(h, T(0), T(0.25), T(0), T(1)) * remap(h, T(0.3), T(0.65), T(1), T(0)));
+  T st = m_fmax(T(0), remap(h, T(0), T(0.1), T(0), T(1)) * remap(h, T(0.2), T(0.3), T(1), T(0)));
   T a = mixT(st, sc, clampT(type * T(2), T(0), T(1)));
   T b = mixT(sc, st, clampT((type - T(0.5)) * T(2), T(0), T(1)));
   (void)cu;
@@ -507,7 +529,7 @@ static v3<T> ray_march(const Frame<T>& f, const Volumes& vol, const Ray<T>& ray,
   T d = dot(ray.d, V(T(0), T(1), T(0)));
   T maxSteps;
   if (f.step_mode == MTC_STEPS_FIXED) maxSteps = (T)f.steps_max;
-  else maxSteps = std::floor(mixT((T)f.steps_min, (T)f.steps_max, T(1) - d));
+  else maxSteps = m_floor(mixT((T)f.steps_min, (T)f.steps_max, T(1) - d));
   T span = end_t - start_t;
   T stepSize = span / maxSteps;
   T transmittance = T(1);
@@ -518,7 +540,7 @@ static v3<T> ray_march(const Frame<T>& f, const Volumes& vol, const Ray<T>& ray,
   T HG_light = hg_modified(cos_angle, f.hg_g, f.hg_si, f.hg_ss);
 
   v3<T> mc;
-  T ax = std::fabs(lightDir.x), ay = std::fabs(lightDir.y), az = std::fabs(lightDir.z);
+  T ax = m_fabs(lightDir.x), ay = m_fabs(lightDir.y), az = m_fabs(lightDir.z);
   if (ax > ay && ax > az) mc = V(ax, T(0), T(0));
   else if (ay > ax && ay > az) mc = V(T(0), ay, T(0));
   else mc = V(T(0), T(0), az);
@@ -535,10 +557,11 @@ static v3<T> ray_march(const Frame<T>& f, const Volumes& vol, const Ray<T>& ray,
   const int cap = 4 * (f.steps_min > f.steps_max ? f.steps_min : f.steps_max) + 64;
   int iter = 0;
   for (T t = start_t; t < end_t && iter < cap; t += stepSize, ++iter) {
+    Region<T>::set(FR_PRIMARY);
     cnt.primary_steps++;
     int idx = pixelID + (int)t;
     T fm = (T)idx;
-    int _index = (int)(fm - T(16) * std::floor(fm / T(16)));  // GLSL mod(x, 16.0)
+    int _index = (int)(fm - T(16) * m_floor(fm / T(16)));  // GLSL mod(x, 16.0)
     T jx, jy;
     jitter_march(f, _index, f.jitter_dim, f.jitter_dim, jx, jy);
     v3<T> jd = V(jx, (jx + jy) * f.jitter_factor, jy);
@@ -559,12 +582,14 @@ static v3<T> ray_march(const Frame<T>& f, const Volumes& vol, const Ray<T>& ray,
       st.lf[0] = (double)lfs.r; st.lf[1] = (double)lfs.g; st.lf[2] = (double)lfs.b; st.lf[3] = (double)lfs.a;
     }
     if (base > T(0)) {
+      Region<T>::set(FR_CLOUD);
       cnt.cloud_steps++;
       T hfd = erode(f, vol, base * f.erosion_scale, sk, h, &cls, &hfs, &ep);
       accum += hfd * f.accum_scale;
       T dal = T(0);
       T terms[6] = {T(0), T(0), T(0), T(0), T(0), T(0)};
       for (int i = 0; i < f.light_samples; ++i) {
+        Region<T>::set(FR_CONE);
         cnt.cone_samples++;
         v3<T> lp = pos + ((stepSize * kern[i]) * (T)i);
         v3<T> slp = rel_pos(f, lp, c);
@@ -575,12 +600,14 @@ static v3<T> ray_march(const Frame<T>& f, const Volumes& vol, const Ray<T>& ray,
           st.cone_pos[i][0] = (double)slp.x; st.cone_pos[i][1] = (double)slp.y; st.cone_pos[i][2] = (double)slp.z;
         }
         if (cb > T(0)) {
+          Region<T>::set(FR_CONE_EROSION);
           cnt.cone_erosions++;
           T e = erode(f, vol, f.cone_erosion_scale * cb, sk, h);
           if (i < 6) terms[i] = e;
           dal += e;
         }
       }
+      Region<T>::set(FR_CLOUD);
       if (rec) {
         // inputs: position in the shell (x, z modulo the weather tile, y
         // modulo the LF tile), height, the cone's first term, sun direction,
@@ -619,6 +646,7 @@ static v3<T> ray_march(const Frame<T>& f, const Volumes& vol, const Ray<T>& ray,
         for (int i = 0; i < 6; ++i) st.cone_e[i] = (double)terms[i];
       }
     }
+    Region<T>::set(FR_PRIMARY);
     if (trace) {
       st.accum = (double)accum;
       st.transmittance = (double)transmittance;
@@ -630,6 +658,7 @@ static v3<T> ray_march(const Frame<T>& f, const Volumes& vol, const Ray<T>& ray,
       break;
     }
   }
+  Region<T>::set(FR_RAY);
   return ret;
 }
 
@@ -638,6 +667,7 @@ template <class T>
 static void march_pixel(const Frame<T>& f, const Volumes& vol, int px, int py, int pixelID, T rgba[4], T mask[4],
                         Counters& cnt, std::vector<LightRecord>* rec = nullptr,
                         std::vector<StepTrace>* trace = nullptr) {
+  Region<T>::set(FR_RAY);
   T ux = (T)px / (T)f.W, uy = (T)py / (T)f.H;
   uy = T(1) - uy;
   v3<T> eye = V(-f.eye_ubo[0], -f.eye_ubo[1], -f.eye_ubo[2]);
@@ -645,7 +675,7 @@ static void march_pixel(const Frame<T>& f, const Volumes& vol, int px, int py, i
   jitter_march(f, pixelID, (T)f.W, (T)f.H, jx, jy);
   Ray<T> ray = cast_ray(f.view, f.tfx, f.tfy, ux, uy, eye, jx, jy);
   T d = dot(V(T(0), T(1), T(0)), ray.d);
-  T bgMul = std::fmax(f.bg_min, d);
+  T bgMul = m_fmax(f.bg_min, d);
   for (int i = 0; i < 4; ++i) mask[i] = T(0);
   if (d < T(0)) {
     v3<T> c1 = V(T(0.0), T(0.16), T(0.51)) * T(0.4);
@@ -671,16 +701,16 @@ static void march_pixel(const Frame<T>& f, const Volumes& vol, int px, int py, i
   T accum = T(0);
   v3<T> col = ray_march(f, vol, ray, c, in.point, in.t, out.t, pixelID, accum, cnt, rec, trace);
   T godAccum = accum;
-  accum = accum * smoothstepT(T(0), T(1), std::fmin(T(1), remap(ray.d.y, f.fade_start, f.fade_end, T(0), T(1))));
+  accum = accum * smoothstepT(T(0), T(1), m_fmin(T(1), remap(ray.d.y, f.fade_start, f.fade_end, T(0), T(1))));
   rgba[0] = mixT(bg.x, col.x, accum);
   rgba[1] = mixT(bg.y, col.y, accum);
   rgba[2] = mixT(bg.z, col.z, accum);
   rgba[3] = T(1);
-  T grey = T(25.0) * std::fmin(T(0.05), T(1) - godAccum);
+  T grey = T(25.0) * m_fmin(T(0.05), T(1) - godAccum);
   encode_float_rgba(grey, mask);
   // :775-779, reachable only when cloud_fade_start < 0.05
   if (ray.d.y < T(0.05)) {
-    for (int i = 0; i < 4; ++i) mask[i] = mask[i] * std::fmax(T(5.0), ray.d.y);
+    for (int i = 0; i < 4; ++i) mask[i] = mask[i] * m_fmax(T(5.0), ray.d.y);
   }
 }
 
@@ -689,6 +719,7 @@ static void march_pixel(const Frame<T>& f, const Volumes& vol, int px, int py, i
 template <class T>
 static void reproject_pixel(const Frame<T>& f, const T* oldview, const float* prev, int x, int y, float out[4],
                             int32_t* idx) {
+  Region<T>::set(FR_REPROJ);
   T ux = (T)x / (T)f.W, uy = (T)y / (T)f.H;
   int pixelID = f.frame;
   v3<T> eye = V(-f.eye_ubo[0], -f.eye_ubo[1], -f.eye_ubo[2]);
@@ -718,10 +749,10 @@ static void reproject_pixel(const Frame<T>& f, const T* oldview, const float* pr
     // round(): ties to even (GLSL leaves ties to the implementation);
     // clamp(ivec2(...), 0, dim-1) done before the conversion so that
     // out-of-range values saturate instead of being undefined.
-    T iu = std::nearbyint((ou - bx) * (T)f.W);
-    T iv = std::nearbyint((ov - by) * (T)f.H);
-    long long sx = (long long)std::fmin(std::fmax(iu, T(0)), (T)(f.W - 1));
-    long long sy = (long long)std::fmin(std::fmax(iv, T(0)), (T)(f.H - 1));
+    T iu = m_nearbyint((ou - bx) * (T)f.W);
+    T iv = m_nearbyint((ov - by) * (T)f.H);
+    long long sx = (long long)m_fmin(m_fmax(iu, T(0)), (T)(f.W - 1));
+    long long sy = (long long)m_fmin(m_fmax(iv, T(0)), (T)(f.H - 1));
     size_t li = (size_t)sy * (size_t)f.W + (size_t)sx;
     if (idx) idx[i] = (int32_t)li;
     if (prev) {

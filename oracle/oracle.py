@@ -49,6 +49,9 @@ def lib() -> C.CDLL:
         h.orc_march_trace.argtypes = [vp, vp, vp, vp, P(A.MtcVolumeDesc), P(A.MtcCamera), P(A.MtcTime), P(A.MtcSun),
                                       P(A.MtcCloudParams), C.c_int, C.c_int, C.c_int, C.c_int, C.c_int, C.c_int64,
                                       vp, vp, vp]
+        h.orc_flop_counts.restype = C.c_int
+        h.orc_flop_counts.argtypes = [vp, vp, vp, vp, P(A.MtcVolumeDesc), P(A.MtcCamera), P(A.MtcCamera), P(A.MtcTime),
+                                      P(A.MtcSun), P(A.MtcCloudParams), C.c_int, C.c_int, vp, C.c_int64, vp, vp]
         h.orc_god_rays.restype = C.c_int
         h.orc_god_rays.argtypes = [P(A.MtcCamera), P(A.MtcSun), C.c_int, C.c_int, vp, vp, vp, C.c_int, vp]
         h.orc_tone_map.restype = C.c_int
@@ -216,6 +219,24 @@ def march_trace(vol: Volumes, cam, time, sun, params, W, H, px, py, precision=64
     if n < 0 or n > cap:
         raise ValueError("orc_march_trace rejected its arguments")
     return buf[: n * 61].view(STEP_TRACE).copy(), rgba, mask
+
+
+FLOP_UNITS = ("ray_marched", "ray_ocean", "ray_sky", "primary_step", "cloud_step", "cone_sample", "cone_erosion",
+              "reproject_pixel")
+
+
+def flop_counts(vol: Volumes, cam, old, time, sun, params, W, H, xy: np.ndarray):
+    """Arithmetic of the GLSL as written, counted by running the march and
+    the reprojection of the listed pixels through the oracle's templates with
+    an operation-counting scalar (oracle/flop_count.hpp): returns
+    {unit: (operations, units)} for the units of FLOP_UNITS."""
+    xy = np.ascontiguousarray(xy, dtype=np.int32).reshape(-1, 2)
+    ops = np.zeros(8, dtype=np.uint64)
+    units = np.zeros(8, dtype=np.uint64)
+    if lib().orc_flop_counts(*vol.ptrs(), C.byref(vol.desc), C.byref(cam), C.byref(old), C.byref(time), C.byref(sun),
+                             C.byref(params), W, H, xy.ctypes.data, xy.shape[0], ops.ctypes.data, units.ctypes.data):
+        raise ValueError("orc_flop_counts rejected its arguments")
+    return {k: (int(o), int(u)) for k, o, u in zip(FLOP_UNITS, ops, units)}
 
 
 def god_rays(cam, sun, rgba: np.ndarray, mask: np.ndarray, precision=32):

@@ -10,6 +10,7 @@
 #include <vector>
 
 #include "cloud_oracle.hpp"
+#include "flop_count.hpp"
 #include "post_oracle.hpp"
 #include "noise_oracle.hpp"
 
@@ -264,6 +265,55 @@ int orc_reprojection_indices(const mtc_camera* cam, const mtc_camera* old, const
         float o[4];
         orc::reproject_pixel<float>(f, ov, nullptr, x, y, o, idx + ((size_t)y * W + x) * N);
       }
+  }
+  return 0;
+}
+
+// Operations per unit of work (flop_count.hpp), summed over the listed
+// pixels marched with the frame's uniforms and reprojected from `old`:
+// ops[8] = per-ray work of marched, ocean and sky-only rays, primary steps,
+// in-cloud steps, cone samples, cone erosions, reprojected pixels; units[8]
+// = how many of each unit that was.  Single-threaded.
+int orc_flop_counts(const void* lf, const void* hf, const uint8_t* curl, const uint8_t* weather,
+                    const mtc_volume_desc* vd, const mtc_camera* cam, const mtc_camera* old, const mtc_time* tm,
+                    const mtc_sun* sun, const mtc_cloud_params* p, int W, int H, const int32_t* xy, int64_t n,
+                    uint64_t* ops, uint64_t* units) {
+  if (!lf || !hf || !curl || !weather || !vd || !cam || !old || !tm || !sun || !p || !xy || !ops || !units ||
+      W <= 0 || H <= 0)
+    return 1;
+  using orc_flop::Flop;
+  using orc_flop::tally;
+  Volumes vol = make_volumes(lf, hf, curl, weather, vd);
+  const int pid = tm->frame_count_mod16;
+  Frame<Flop> f = orc::make_frame<Flop>(*cam, *tm, *sun, *p, W, H);
+  Flop ov[16];
+  for (int i = 0; i < 16; ++i) ov[i] = Flop(old->view[i]);
+  std::vector<float> prev((size_t)W * H * 4, 0.0f);
+  for (int k = 0; k < 8; ++k) ops[k] = units[k] = 0;
+  for (int64_t i = 0; i < n; ++i) {
+    const int x = xy[2 * i], y = xy[2 * i + 1];
+    tally = orc_flop::Tally();
+    Counters cnt;
+    Flop rgba[4], mask[4];
+    orc::march_pixel<Flop>(f, vol, x, y, pid, rgba, mask, cnt);
+    const int kind = cnt.rays_marched ? 0 : (cnt.rays_ocean ? 1 : 2);
+    ops[kind] += tally.ops[orc::FR_RAY];
+    units[kind] += 1;
+    ops[3] += tally.ops[orc::FR_PRIMARY];
+    units[3] += cnt.primary_steps;
+    ops[4] += tally.ops[orc::FR_CLOUD];
+    units[4] += cnt.cloud_steps;
+    ops[5] += tally.ops[orc::FR_CONE];
+    units[5] += cnt.cone_samples;
+    ops[6] += tally.ops[orc::FR_CONE_EROSION];
+    units[6] += cnt.cone_erosions;
+    tally = orc_flop::Tally();
+    float o[4];
+    int32_t taps[64];
+    if (p->motion_blur_samples > 64) return 1;
+    orc::reproject_pixel<Flop>(f, ov, prev.data(), x, y, o, taps);
+    ops[7] += tally.ops[orc::FR_REPROJ];
+    units[7] += 1;
   }
   return 0;
 }

@@ -1,7 +1,6 @@
 """The bench's command-line contract, on the CPU: more than one GPU and
 invalid step counts are refused with one JSON line and exit status 2 before
-any device work, and the FLOP weights of the roofline are the documented
-ones."""
+any device work (the roofline's FLOP weights: tests/test_flop_weights.py)."""
 import json
 import os
 import subprocess
@@ -30,19 +29,6 @@ def test_invalid_step_counts_are_refused(args):
     r = _bench(*args)
     assert r.returncode == 2
     assert "error" in json.loads(r.stdout.strip().splitlines()[-1])
-
-
-def test_march_flops_weights_match_the_documented_table():
-    sys.path.insert(0, ROOT)
-    import bench
-
-    assert bench.FLOPS["ray_marched"] == 500
-    assert bench.FLOPS["primary_step"] == 289
-    assert bench.FLOPS["cloud_step"] == 211
-    assert bench.FLOPS["cone_sample"] == 222
-    one_of_each = {"rays_marched": 1, "rays_ocean": 0, "rays_sky": 0, "primary_steps": 1, "cloud_steps": 1,
-                   "cone_samples": 6, "cone_erosions": 0}
-    assert bench.march_flops(one_of_each) == 500 + 289 + 211 + 6 * 222
 
 
 @pytest.mark.gpu

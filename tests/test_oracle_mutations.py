@@ -95,7 +95,7 @@ def _build(work, name, mutation):
     src = os.path.join(d, "oracle")
     os.makedirs(src)
     shutil.copytree(os.path.join(ROOT, "include"), os.path.join(d, "include"))
-    for f in ("oracle_capi.cpp", "cloud_oracle.hpp", "noise_oracle.hpp", "post_oracle.hpp"):
+    for f in ("oracle_capi.cpp", "cloud_oracle.hpp", "flop_count.hpp", "noise_oracle.hpp", "post_oracle.hpp"):
         shutil.copy(os.path.join(ORACLE, f), src)
     if mutation is not None:
         fname, text, repl = mutation
