@@ -39,8 +39,8 @@ MUTATIONS = [
     ("cone point divided by 8", "cloud_oracle.hpp",
      "v3<T> slp = rel_pos(f, lp, c);", "v3<T> slp = rel_pos(f, lp, c) / f.sample_divisor;", [CONE]),
     ("cone sum without term 0", "cloud_oracle.hpp",
-     "for (int i = 0; i < f.light_samples; ++i) {\n        cnt.cone_samples++;",
-     "for (int i = 1; i < f.light_samples; ++i) {\n        cnt.cone_samples++;", [CLOSED]),
+     "for (int i = 0; i < f.light_samples; ++i) {\n        Region<T>::set(FR_CONE);",
+     "for (int i = 1; i < f.light_samples; ++i) {\n        Region<T>::set(FR_CONE);", [CLOSED]),
     ("cone erosion at the primary scale", "cloud_oracle.hpp",
      "T e = erode(f, vol, f.cone_erosion_scale * cb, sk, h);", "T e = erode(f, vol, f.erosion_scale * cb, sk, h);",
      [CLOSED, TERMS]),
@@ -59,7 +59,7 @@ MUTATIONS = [
      [CLOSED, TERMS]),
     ("no early exit", "cloud_oracle.hpp", "if (accum >= T(1)) {", "if (accum >= T(2)) {", [CLOSED]),
     ("relative height without abs", "cloud_oracle.hpp",
-     "T num = std::fabs(cosT * (lr - ls));", "T num = cosT * (lr - ls);", [GEOM]),
+     "T num = m_fabs(cosT * (lr - ls));", "T num = cosT * (lr - ls);", [GEOM]),
     ("relative position without the inner offset", "cloud_oracle.hpp",
      "return (pos - V(c.x, f.inner_off, c.z)) / f.thickness;", "return (pos - V(c.x, T(0), c.z)) / f.thickness;",
      [GEOM]),
