@@ -104,10 +104,15 @@ def make_sequence(n_frames: int):
     return seq
 
 
-KERNELS = {"plain": A.MTC_KERNEL_PLAIN, "fast": A.MTC_KERNEL_FAST, "precision": A.MTC_KERNEL_PRECISION,
-           "learned": A.MTC_KERNEL_LEARNED}
+# name -> (kernel, context flags); the *_bricks contexts read the primary
+# steps' LF samples through LDS-staged bricks (bit-identical results)
+KERNELS = {"plain": (A.MTC_KERNEL_PLAIN, 0), "fast": (A.MTC_KERNEL_FAST, 0),
+           "fast_bricks": (A.MTC_KERNEL_FAST, A.MTC_FLAG_LDS_BRICKS), "precision": (A.MTC_KERNEL_PRECISION, 0),
+           "precision_bricks": (A.MTC_KERNEL_PRECISION, A.MTC_FLAG_LDS_BRICKS), "learned": (A.MTC_KERNEL_LEARNED, 0)}
 DTYPES = {"plain": "fp32", "fast": "fp32 math on fp16 texels, fp64 shell",
+          "fast_bricks": "fp32 math on fp16 texels, fp64 shell",
           "precision": "fp16 filtering, fp32 accumulation, fp64 shell",
+          "precision_bricks": "fp16 filtering, fp32 accumulation, fp64 shell",
           "learned": "fast + bf16 MFMA light network (fp32 accumulation)"}
 _NET = None
 PSNR_BAR_DB = 45.0
@@ -260,12 +265,12 @@ def run_batch(kernel: int, renders: int = 5, check_rays: int = 0) -> dict:
 POST_BYTES_PER_PX = {"god_rays": 80, "tone_map": 20, "txaa": 12}
 
 
-def run_post(kernel: int, seq, warmup: int, reps: int = 100) -> dict:
+def run_post(kernel, seq, warmup: int, reps: int = 100) -> dict:
     """The post passes at the workload's size after the warm-up frames of the
     sequence: god rays, tone map (RGBA8), TXAA, each issued reps times back
     to back between synchronisations (wall time per pass, so a pass's launch
     gap is included)."""
-    st, _ = run_gpu(kernel, seq, warmup, 0)
+    st, _ = run_gpu(kernel[0], seq, warmup, 0, flags=kernel[1])
     params, sun = M.preset_params(PRESET)
     cur, old, tm = seq[warmup - 1]
     passes = (("god_rays", lambda: st.god_rays(cur, sun)), ("tone_map", lambda: st.tone_map(tm)),
@@ -291,7 +296,7 @@ def run_post(kernel: int, seq, warmup: int, reps: int = 100) -> dict:
 POSES_4K = [(yaw, elev) for yaw in (0.0, 90.0, 180.0, 270.0) for elev in (0.0, 20.0, 40.0, 60.0)]
 
 
-def run_4k(kernel: int, frames_per_pose: int = 4, check_rays: int = 0) -> dict:
+def run_4k(kernel, frames_per_pose: int = 4, check_rays: int = 0) -> dict:
     """The high-resolution precision workload: 3840x2160, 192 primary steps
     (preset 3), LF 256^3 and HF 64^3 FP16 volumes, 16 ground poses (yaw 0 /
     90 / 180 / 270 x elevation 0 / 20 / 40 / 60 deg), each marched as
@@ -314,8 +319,8 @@ def run_4k(kernel: int, frames_per_pose: int = 4, check_rays: int = 0) -> dict:
             old = frames[-1][0]
         seqs.append(frames)
     ms, errs = {}, []
-    for name, k in (("plain", A.MTC_KERNEL_PLAIN), ("kernel", kernel)):
-        with M.CloudStage(W4, H4, M.VolumeSpec(256, 64, 128, 512, A.MTC_FMT_FP16), kernel=k) as st:
+    for name, (k, fl) in (("plain", (A.MTC_KERNEL_PLAIN, 0)), ("kernel", kernel)):
+        with M.CloudStage(W4, H4, M.VolumeSpec(256, 64, 128, 512, A.MTC_FMT_FP16), kernel=k, flags=fl) as st:
             st.generate_volumes(1, 2)
             for cur, old, tm in seqs[0]:
                 st.frame(cur, old, tm, sun, params)
@@ -449,6 +454,9 @@ def main() -> int:
     ap.add_argument("--no-batch", action="store_true", help="skip the 96-view batch measurement")
     ap.add_argument("--no-4k", action="store_true", help="skip the 4K / 192-step 16-pose measurement")
     ap.add_argument("--no-post", action="store_true", help="skip the post-pass (god rays, tone map, TXAA) timing")
+    ap.add_argument("--only-4k", action="store_true",
+                    help="run only the 4K / 192-step 16-pose workload with --kernel (precision when auto) and print "
+                         "its JSON; no checker (for counter profiles)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(json.dumps({"error": "the cloud stage runs on one GPU; only --gpus 1 is supported",
@@ -459,17 +467,24 @@ def main() -> int:
         return 2
     global LIGHT_STEPS
     LIGHT_STEPS = max(1, args.light_steps)
+    if args.only_4k:
+        name = "precision" if args.kernel == "auto" else args.kernel
+        if name in ("plain", "learned"):
+            print(json.dumps({"error": "--only-4k takes a fast or precision kernel"}))
+            return 2
+        print(json.dumps({"kernel": name, "precision_4k_16_poses": run_4k(KERNELS[name])}))
+        return 0
     seq = make_sequence(args.warmup + args.steps)
     work = count_work(seq, args.warmup, args.steps)
     names = sorted(KERNELS, key=lambda n: KERNELS[n]) if args.kernel == "auto" else [args.kernel]
 
     results = {}
     for name in names:
-        k = KERNELS[name]
+        k, kflags = KERNELS[name]
         # one timed frame in TIMING_STRIDE carries hipEvents attached to each
         # kernel's dispatch on its own stream (MTC_FLAG_TIMING leaves the
         # frame's structure unchanged)
-        st, dt = run_gpu(k, seq, args.warmup, args.steps, flags=A.MTC_FLAG_TIMING)
+        st, dt = run_gpu(k, seq, args.warmup, args.steps, flags=A.MTC_FLAG_TIMING | kflags)
         kt = st.timings()
         rgba, _ = st.read_output()
         vols = st.get_volumes()
@@ -487,7 +502,7 @@ def main() -> int:
     if args.no_cpu_baseline:
         # no checker in this run: pick among the kernels the GPU tests hold to
         # the PSNR bar (tests/test_gpu_fast.py)
-        valid = [n for n in names if n in ("fast", "precision")]
+        valid = [n for n in names if n in ("fast", "precision", "fast_bricks", "precision_bricks")]
     else:
         valid = [n for n in names if results[n]["acc"]["psnr_db_fp64"] >= PSNR_BAR_DB]
     if not valid:
@@ -515,7 +530,7 @@ def main() -> int:
             "algorithmic_bytes": alg_bytes,
             "traffic_over_algorithmic": round(traffic / alg_bytes, 2) if traffic else None,
             "flops_per_launch": flops_per_launch, "avg_launch_ms": round(march_ms, 4)}
-    if best in ("fast", "precision"):
+    if best != "plain" and best != "learned":
         roof["launch"] = "march tiles + the frame's reprojection strips in one launch (its duration covers both)"
     reproj_bytes = W * H * 16 * 2
     src = results.get("plain", r)["kt"]
@@ -557,13 +572,13 @@ def main() -> int:
         "roofline": roof,
         "accuracy": r["acc"],
     }
-    if not args.no_batch and best in ("fast", "precision"):
-        out["batch_96_views"] = run_batch(KERNELS[best], check_rays=0 if args.no_cpu_baseline else 500)
+    if not args.no_batch and best not in ("plain", "learned"):
+        out["batch_96_views"] = run_batch(KERNELS[best][0], check_rays=0 if args.no_cpu_baseline else 500)
         print(f"[bench] 96-view batch: {out['batch_96_views']}", file=sys.stderr, flush=True)
     if not args.no_post:
         out["post_passes_1080p"] = run_post(KERNELS[best], seq, args.warmup)
         print(f"[bench] post passes: {out['post_passes_1080p']}", file=sys.stderr, flush=True)
-    if not args.no_4k and best in ("fast", "precision"):
+    if not args.no_4k and best not in ("plain", "learned"):
         out["precision_4k_16_poses"] = run_4k(KERNELS[best], check_rays=0 if args.no_cpu_baseline else 1500)
         print(f"[bench] 4K / 192 steps, 16 poses: {out['precision_4k_16_poses']}", file=sys.stderr, flush=True)
     if not args.no_cpu_baseline:

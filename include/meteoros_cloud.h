@@ -221,8 +221,13 @@ enum {
 enum {
   MTC_FLAG_COUNTERS = 1,     /* accumulate mtc_counters with device atomics  */
   MTC_FLAG_FULL_MARCH = 2,   /* march every pixel (no 4x4 selection)          */
-  MTC_FLAG_TIMING = 4        /* hipEvents around each kernel on its stream
+  MTC_FLAG_TIMING = 4,       /* hipEvents around each kernel on its stream
                                 (mtc_get_timings); frames keep their structure */
+  MTC_FLAG_LDS_BRICKS = 8    /* fast / precision frames, 8 lanes per ray: the
+                                primary steps' LF samples go through an 8^3
+                                texel box per wave staged in LDS from a 4^3-
+                                bricked copy of the LF (+4 bytes per LF texel);
+                                results bit-identical to the default path */
 };
 
 typedef struct mtc_desc {
@@ -251,6 +256,8 @@ typedef struct mtc_counters {
   uint64_t mlp_audits;       /* network answers also cone-marched     */
   uint64_t mlp_audit_sq_err; /* sum over audits of (network - cone)^2
                                 optical depth, in units of 1e-6       */
+  uint64_t lf_brick_stages;  /* MTC_FLAG_LDS_BRICKS: LDS box stagings  */
+  uint64_t lf_brick_samples; /* primary LF samples read from LDS       */
 } mtc_counters;
 
 /* Kernel time accumulated with hipEvents on the stream each kernel runs

@@ -33,6 +33,7 @@ MTC_KERNEL_LEARNED = 3
 MTC_FLAG_COUNTERS = 1
 MTC_FLAG_FULL_MARCH = 2
 MTC_FLAG_TIMING = 4
+MTC_FLAG_LDS_BRICKS = 8
 MTC_DISPLAY_RGBA8 = 0
 MTC_DISPLAY_RGBA32F = 1
 
@@ -95,7 +96,8 @@ class MtcDesc(C.Structure):
 class MtcCounters(C.Structure):
     _fields_ = [(n, C.c_uint64) for n in (
         "rays_marched", "rays_ocean", "rays_sky", "primary_steps", "cloud_steps", "cone_samples",
-        "cone_erosions", "early_exits", "mlp_queries", "mlp_fallbacks", "mlp_audits", "mlp_audit_sq_err")]
+        "cone_erosions", "early_exits", "mlp_queries", "mlp_fallbacks", "mlp_audits", "mlp_audit_sq_err",
+        "lf_brick_stages", "lf_brick_samples")]
 
 
 class MtcTimings(C.Structure):

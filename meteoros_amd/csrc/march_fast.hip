@@ -72,6 +72,7 @@ __device__ __forceinline__ uint4 entry(const uint4* __restrict__ q, uint32_t i) 
 struct Coord {
   uint32_t i0;   // linear index of the (x0, y0, z0) texel
   float ax, ay, az;
+  uint32_t x0, y0, z0;
 };
 
 // s * N - 0.5 as one fma: s * N is exact (N a power of two), so this is
@@ -87,6 +88,9 @@ __device__ __forceinline__ Coord coord3(int sh, f3 s) {
   c.az = uz - fz;
   const uint32_t x0 = (uint32_t)(int)fx & m, y0 = (uint32_t)(int)fy & m, z0 = (uint32_t)(int)fz & m;
   c.i0 = (((z0 << sh) + y0) << sh) + x0;
+  c.x0 = x0;
+  c.y0 = y0;
+  c.z0 = z0;
   return c;
 }
 
@@ -96,11 +100,11 @@ __device__ __forceinline__ Coord coord3(int sh, f3 s) {
 // than the 16-byte 2x2 (x, y) entries, although the table doubles (64 MiB
 // at 128^3): the march is limited by L2 request throughput more than by
 // capacity.
+//
+// filter_lf: the trilinear of one footprint, a = the z0 slice's texels
+// (x0y0, x1y0, x0y1, x1y1), b = the z1 slice's, each a half2 (r, fbm).
 template <bool F16>
-__device__ __forceinline__ float2 sample_lf(const uint4* __restrict__ q, int sh, f3 s) {
-  Coord c = coord3(sh, s);
-  uint4 a = entry(q, 2u * c.i0);
-  uint4 b = entry(q, 2u * c.i0 + 1u);
+__device__ __forceinline__ float2 filter_lf(const Coord& c, uint4 a, uint4 b) {
   if (F16) {
     h2v wx = {(_Float16)c.ax, (_Float16)c.ax}, wy = {(_Float16)c.ay, (_Float16)c.ay},
         wz = {(_Float16)c.az, (_Float16)c.az};
@@ -112,6 +116,12 @@ __device__ __forceinline__ float2 sample_lf(const uint4* __restrict__ q, int sh,
   float2 a0 = lerp2(f2(a.x), f2(a.y), c.ax), a1 = lerp2(f2(a.z), f2(a.w), c.ax);
   float2 b0 = lerp2(f2(b.x), f2(b.y), c.ax), b1 = lerp2(f2(b.z), f2(b.w), c.ax);
   return lerp2(lerp2(a0, a1, c.ay), lerp2(b0, b1, c.ay), c.az);
+}
+
+template <bool F16>
+__device__ __forceinline__ float2 sample_lf(const uint4* __restrict__ q, int sh, f3 s) {
+  const Coord c = coord3(sh, s);
+  return filter_lf<F16>(c, entry(q, 2u * c.i0), entry(q, 2u * c.i0 + 1u));
 }
 
 // HF: single channel, the full 2x2x2 footprint in one entry
@@ -253,6 +263,9 @@ struct FastViews {
   unsigned int calib_mask;    // bins whose held-out error is too high
   float fallback_density;     // local density above which the cone runs
   int audit;                  // also cone-march one in `audit` answered steps (power of two, 0 = off)
+  // LDS-staged LF bricks (MTC_FLAG_LDS_BRICKS): the LF's half2 (r, fbm)
+  // texels in 4^3 bricks of 256 bytes (pack_lf_brick_kernel)
+  const uint4* lf_brick;
 };
 
 // Shell entry/exit of a ray (raySphereIntersection, :229-273, with the
@@ -332,9 +345,19 @@ struct Prim {
   float da;         // accumulated-density increment of the step
 };
 
+// What eval_primary_pre hands to the LF fetch and eval_primary_post: the
+// skewed sample point and the coverage terms of a step that needs its LF
+// sample (need_lf false: the step's density is known to be 0 without it).
+struct PrimLf {
+  bool need_lf;
+  f3 sk;
+  float cov, gw;
+};
+
+// A primary step (:627-650) up to its LF fetch.
 template <int COV, bool F16>
-__device__ __forceinline__ Prim eval_primary(const FrameConsts& fc, const FastViews& fv, const float* sjit,
-                                             const RayMarch& rm, int k) {
+__device__ __forceinline__ Prim eval_primary_pre(const FrameConsts& fc, const FastViews& fv, const float* sjit,
+                                                 const RayMarch& rm, int k, PrimLf& pl) {
   Prim o;
   o.cloud = false;
   o.da = 0.0f;
@@ -344,6 +367,8 @@ __device__ __forceinline__ Prim eval_primary(const FrameConsts& fc, const FastVi
   o.h = 0.0f;
   o.base = 0.0f;
   o.rel = mk(0.0f, 0.0f, 0.0f);
+  pl.need_lf = false;
+  pl.sk = mk(0.0f, 0.0f, 0.0f);
   const float t = __fmaf_rn((float)k, rm.stepSize, rm.start_t);
   o.valid = (t < rm.end_t) && (k < fc.mk_cap);
   if (!o.valid) return o;
@@ -383,12 +408,26 @@ __device__ __forceinline__ Prim eval_primary(const FrameConsts& fc, const FastVi
     gw = gradient_weight(w.y, sc, st);
     if (gw <= cov) return o;
   }
-  const float base = density<COV>(sample_lf<F16>(fv.lf, fv.lf_sh, sk), cov, gw) * fc.base_density_factor;
+  pl.need_lf = true;
+  pl.sk = sk;
+  pl.cov = cov;
+  pl.gw = gw;
+  return o;
+}
+
+// The rest of a primary step that needed its LF sample `lf`: density,
+// and the erosion of an in-cloud step.
+template <int COV, bool F16>
+__device__ __forceinline__ void eval_primary_post(const FrameConsts& fc, const FastViews& fv, const PrimLf& pl,
+                                                  float2 lf, Prim& o) {
+  const float base = density<COV>(lf, pl.cov, pl.gw) * fc.base_density_factor;
   o.base = base;
   if (base > 0.0f) {
     o.cloud = true;
     // erodeCloudWithHighFrequency at (sk, h): one curl + HF fetch serves the
     // primary erosion and all cone erosions of the step (:645, :665)
+    const f3 sk = pl.sk;
+    const float h = o.h;
     float2 cn = sample_map<F16>(fv.curl, fv.curl_sh, sk.x, sk.y, 1.0f);
     float kk = (1.0f - h) * 0.5f;
     f3 p2 = mk(sk.x + cn.x * kk, sk.y + cn.y * kk, sk.z);
@@ -398,7 +437,129 @@ __device__ __forceinline__ Prim eval_primary(const FrameConsts& fc, const FastVi
     o.inv1m = rcp(1.0f - o.m);
     o.da = ((base * fc.erosion_scale - o.m) * o.inv1m) * fc.accum_scale;
   }
+}
+
+template <int COV, bool F16>
+__device__ __forceinline__ Prim eval_primary(const FrameConsts& fc, const FastViews& fv, const float* sjit,
+                                             const RayMarch& rm, int k) {
+  PrimLf pl;
+  Prim o = eval_primary_pre<COV, F16>(fc, fv, sjit, rm, k, pl);
+  if (pl.need_lf) eval_primary_post<COV, F16>(fc, fv, pl, sample_lf<F16>(fv.lf, fv.lf_sh, pl.sk), o);
   return o;
+}
+
+// ---------------------------------------------------------------------------
+// LDS-staged LF bricks (MTC_FLAG_LDS_BRICKS)
+//
+// The primary steps of one wave (8 rays x 8 consecutive steps at lanes per
+// ray 8) sample the LF at the skewed point, one eighth of the cone's
+// frequency (:635), so on the steeper rays they stay within a few texels of
+// each other for several chunks.  Each wave keeps an 8^3-texel box of the
+// LF's half2 (r, fbm) texels in LDS (2 KiB), origin aligned to 4 texels,
+// staged from the 4^3-bricked copy of the volume: the box is two bricks on
+// each axis, and each of the 64 lanes loads 32 contiguous bytes of it.  A
+// sample whose 2x2x2 footprint lies in the box (base texel within 0..6 of
+// the origin on every axis, REPEAT wrap as a mask) reads its eight texels
+// from LDS (ds_read2 pairs); the others read the footprint table.  Both
+// hold the same half2 values and feed the same filter_lf, so the results
+// are bit-identical to the table path.  The box moves (re-staged around the
+// first sample that misses it) only when that serves at least
+// kBrickMinGain more of the chunk's samples than the current box does.
+// Cone samples (full frequency, scattered) stay on the table path.
+// ---------------------------------------------------------------------------
+constexpr int kBrickMinGain = 8;
+
+struct BrickBox {
+  int ox, oy, oz;  // texel origin of the staged box (multiples of 4); wave-uniform
+  bool valid;
+};
+
+// Stage the box at (ox, oy, oz) into `box` (512 half2 texels, x fastest,
+// then y, then z): lane l loads piece l & 7 (two 4-texel rows) of brick
+// l >> 3 of the 2 x 2 x 2 bricks the box spans.
+__device__ __forceinline__ void stage_brick_box(const uint4* __restrict__ bricks, int sh, int ox, int oy, int oz,
+                                                uint32_t* __restrict__ box) {
+  const int lane = threadIdx.x & 63;
+  const int nbsh = sh - 2;
+  const uint32_t nbm = (1u << nbsh) - 1u;
+  const int b = lane >> 3, p = lane & 7;
+  const uint32_t bx = ((uint32_t)(ox >> 2) + (uint32_t)(b & 1)) & nbm;
+  const uint32_t by = ((uint32_t)(oy >> 2) + (uint32_t)((b >> 1) & 1)) & nbm;
+  const uint32_t bz = ((uint32_t)(oz >> 2) + (uint32_t)(b >> 2)) & nbm;
+  const uint32_t brick = (((bz << nbsh) + by) << nbsh) + bx;
+  // brick texel (x, y, z) at x + 4 y + 16 z: piece p = rows (y, y + 1) of
+  // slice z, y = 2 (p & 1), z = p >> 1
+  const uint4 r0 = entry(bricks, brick * 16u + 2u * (uint32_t)p);
+  const uint4 r1 = entry(bricks, brick * 16u + 2u * (uint32_t)p + 1u);
+  const int zl = 4 * (b >> 2) + (p >> 1), yl = 4 * ((b >> 1) & 1) + 2 * (p & 1), xl = 4 * (b & 1);
+  *(uint4*)(box + xl + 8 * yl + 64 * zl) = r0;
+  *(uint4*)(box + xl + 8 * (yl + 1) + 64 * zl) = r1;
+}
+
+// The LF sample of every lane with `need` set, through the wave's LDS box
+// where the footprint fits, else through the footprint table.  Called by
+// all 64 lanes of the wave together (ballots, staging); lanes without
+// `need` return an unused value.  `staged` / `served` count box stagings
+// and samples read from LDS (counting builds).
+template <bool F16>
+__device__ __forceinline__ float2 brick_sample_lf(const FastViews& fv, uint32_t* __restrict__ box, BrickBox& bb,
+                                                 bool need, f3 sk, unsigned long long& staged,
+                                                 unsigned long long& served) {
+  const unsigned long long needb = __ballot(need);
+  if (!needb) return make_float2(0.0f, 0.0f);  // no LF sample in this chunk (most chunks)
+  const int sh = fv.lf_sh;
+  const uint32_t m = (1u << sh) - 1u;
+  const Coord c = coord3(sh, sk);
+  // offset of the footprint's base texel in a box at (ox, oy, oz); true when
+  // the whole 2x2x2 footprint lies in the box
+  auto in_box = [&](int ox, int oy, int oz, uint32_t& off) {
+    const uint32_t dx = (c.x0 - (uint32_t)ox) & m, dy = (c.y0 - (uint32_t)oy) & m, dz = (c.z0 - (uint32_t)oz) & m;
+    off = dx + 8u * dy + 64u * dz;
+    return dx < 7u && dy < 7u && dz < 7u;
+  };
+  uint32_t off = 0u;
+  bool inb = need && bb.valid && in_box(bb.ox, bb.oy, bb.oz, off);
+  const unsigned long long inbb = __ballot(inb);
+  const unsigned long long miss = needb & ~inbb;
+  if (miss) {
+    // candidate box: one texel behind the first missing sample, aligned to 4
+    const int l = __builtin_ctzll(miss);
+    const uint32_t rx = (uint32_t)__builtin_amdgcn_readlane((int)c.x0, l);
+    const uint32_t ry = (uint32_t)__builtin_amdgcn_readlane((int)c.y0, l);
+    const uint32_t rz = (uint32_t)__builtin_amdgcn_readlane((int)c.z0, l);
+    const int cx = (int)(((rx - 1u) & m) & ~3u), cy = (int)(((ry - 1u) & m) & ~3u), cz = (int)(((rz - 1u) & m) & ~3u);
+    uint32_t off2 = 0u;
+    const bool fit = need && in_box(cx, cy, cz, off2);
+    const unsigned long long fitb = __ballot(fit);
+    if (__popcll(fitb) >= __popcll(inbb) + kBrickMinGain) {
+      // the previous chunk's reads of the box are done before it is rewritten
+      __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+      __builtin_amdgcn_wave_barrier();
+      __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+      stage_brick_box(fv.lf_brick, sh, cx, cy, cz, box);
+      __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+      __builtin_amdgcn_wave_barrier();
+      __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+      bb.ox = cx;
+      bb.oy = cy;
+      bb.oz = cz;
+      bb.valid = true;
+      off = off2;
+      inb = fit;
+      if ((threadIdx.x & 63) == 0) staged++;
+    }
+  }
+  uint4 a = make_uint4(0u, 0u, 0u, 0u), b = a;
+  if (inb) {
+    const uint32_t* L = box + off;
+    a = make_uint4(L[0], L[1], L[8], L[9]);
+    b = make_uint4(L[64], L[65], L[72], L[73]);
+    served++;
+  } else if (need) {
+    a = entry(fv.lf, 2u * c.i0);
+    b = entry(fv.lf, 2u * c.i0 + 1u);
+  }
+  return filter_lf<F16>(c, a, b);
 }
 
 // Per-wave LDS work area of the compacted cone march.
@@ -493,11 +654,14 @@ __device__ __forceinline__ uint32_t step_hash(uint32_t a, uint32_t b, uint32_t c
 // LF fetches): two dependent round trips per pair instead of four, for ~18
 // more VGPRs (the frame kernel runs 5 waves per SIMD with it, faster than
 // 6 without; the batch and learned kernels keep single rounds).
-template <int COV, bool F16, bool FP64SHELL, bool COUNT, int G, bool LEARN = false, bool PAIR = false>
+// BRICK: primary LF samples through the wave's LDS box (brick_sample_lf).
+template <int COV, bool F16, bool FP64SHELL, bool COUNT, int G, bool LEARN = false, bool PAIR = false,
+          bool BRICK = false>
 __device__ __forceinline__ void march_tile(const FrameConsts& fc, const FastViews& fv, float4* __restrict__ out_color,
                                            float4* __restrict__ out_mask, unsigned long long* counters, int full,
                                            int rx0, int ry0, int tile_x, const float* sjit, const float* skern,
-                                           ConeLds* cone_lds, uint16_t* mlp_lds = nullptr) {
+                                           ConeLds* cone_lds, uint16_t* mlp_lds = nullptr,
+                                           uint32_t* lf_box = nullptr) {
   const int lane = threadIdx.x & 63;
   const int wave = threadIdx.x >> 6;
   const int slot = threadIdx.x / G;  // ray slot in the tile
@@ -522,7 +686,9 @@ __device__ __forceinline__ void march_tile(const FrameConsts& fc, const FastView
               kz = cone_lane ? skern[3 * ci + 2] : 0.0f;
   unsigned long long n_steps = 0, n_cloud = 0, n_cone = 0, n_cone_er = 0, n_early = 0;
   unsigned long long n_ocean = 0, n_sky = 0, n_marched = 0, n_mlp = 0, n_mlp_fb = 0, n_audit = 0;
+  unsigned long long n_bstage = 0, n_bserved = 0;
   float audit_sq = 0.0f;
+  BrickBox bb{0, 0, 0, false};
 
   // --- per-ray set-up (main, :690-757) --------------------------------------
   RayMarch rm;
@@ -561,7 +727,18 @@ __device__ __forceinline__ void march_tile(const FrameConsts& fc, const FastView
     pr.valid = false;
     pr.cloud = false;
     pr.da = 0.0f;
-    if (!gdone) pr = eval_primary<COV, F16>(fc, fv, sjit, rm, k0 + j);
+    if (BRICK) {
+      // the LF fetch runs with all 64 lanes (ballots, box staging)
+      PrimLf pl;
+      pl.need_lf = false;
+      pl.sk = mk(0.0f, 0.0f, 0.0f);
+      pl.cov = pl.gw = 0.0f;
+      if (!gdone) pr = eval_primary_pre<COV, F16>(fc, fv, sjit, rm, k0 + j, pl);
+      const float2 lf = brick_sample_lf<F16>(fv, lf_box, bb, pl.need_lf, pl.sk, n_bstage, n_bserved);
+      if (pl.need_lf) eval_primary_post<COV, F16>(fc, fv, pl, lf, pr);
+    } else if (!gdone) {
+      pr = eval_primary<COV, F16>(fc, fv, sjit, rm, k0 + j);
+    }
 
     // compacted cone march of this chunk's in-cloud steps
     float E = 0.0f;
@@ -809,6 +986,10 @@ __device__ __forceinline__ void march_tile(const FrameConsts& fc, const FastView
       wave_count(counters + 10, n_audit);
       wave_count(counters + 11, (unsigned long long)rintf(audit_sq * 1.0e6f));
     }
+    if (BRICK) {
+      wave_count(counters + 12, n_bstage);
+      wave_count(counters + 13, n_bserved);
+    }
   }
 }
 
@@ -857,7 +1038,7 @@ __device__ __forceinline__ void file_tile(const TileOrder& to, uint32_t tile, ui
 // they run in the wave slots the draining march leaves free, and the frame
 // needs no second stream and no fork / join events.  The two parts write
 // disjoint pixels and the march reads nothing the reprojection writes.
-template <int COV, bool F16, bool FP64SHELL, bool COUNT, int G>
+template <int COV, bool F16, bool FP64SHELL, bool COUNT, int G, bool BRICK>
 __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(COUNT ? 1 : 5))) void march_kernel(
     FrameConsts fc, FastViews fv, OutputViews ov, int full, int tiles_x, int n_tiles) {
   const int b = (int)blockIdx.x;
@@ -872,13 +1053,14 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(COUNT ? 1 : 
   __shared__ float sjit[48];
   __shared__ float skern[18];
   __shared__ ConeLds cone_lds[1];
+  __shared__ __align__(16) uint32_t lf_box[BRICK ? 512 : 4];
   if (threadIdx.x < 48) sjit[threadIdx.x] = (&fc.step_jitter[0][0])[threadIdx.x];
   if (threadIdx.x < 18) skern[threadIdx.x] = (&fc.kern[0][0])[threadIdx.x];
   __syncthreads();
   constexpr int TX = 8, TY = 64 / G / 8;
-  march_tile<COV, F16, FP64SHELL, COUNT, G, false, true>(fc, fv, ov.color, ov.mask, ov.counters, full,
-                                                         (b % tiles_x) * TX, (b / tiles_x) * TY, TX, sjit, skern,
-                                                         cone_lds);
+  march_tile<COV, F16, FP64SHELL, COUNT, G, false, true, BRICK>(fc, fv, ov.color, ov.mask, ov.counters, full,
+                                                                (b % tiles_x) * TX, (b / tiles_x) * TY, TX, sjit,
+                                                                skern, cone_lds, nullptr, lf_box);
 }
 
 // One frame with learned light (fast variant's sampling): the in-cloud
@@ -1019,6 +1201,14 @@ __device__ __forceinline__ void texel4(const void* src, int fmt, size_t idx, flo
   }
 }
 
+// One LF texel as half2 (r, .625g + .25b + .125a); both LF copies (the
+// footprint table and the bricked copy) hold exactly these values.
+__device__ __forceinline__ uint32_t lf_texel_h2(const void* src, int fmt, int sh, int x, int y, int z) {
+  float v[4];
+  texel4(src, fmt, (((size_t)z << sh) + y) * ((size_t)1 << sh) + x, v);
+  return pack_h2(v[0], (v[1] * 0.625f + v[2] * 0.25f) + v[3] * 0.125f);
+}
+
 // LF -> (r, .625g + .25b + .125a) at the 2x2x2 footprint of each texel:
 // entry 2 idx holds the z0 slice's 2x2, entry 2 idx + 1 the z1 slice's
 __global__ __launch_bounds__(256) void pack_lf_kernel(const void* src, int fmt, int sh, uint4* out) {
@@ -1027,14 +1217,22 @@ __global__ __launch_bounds__(256) void pack_lf_kernel(const void* src, int fmt, 
   if (idx >= (size_t)N * N * N) return;
   int x = (int)(idx & m), y = (int)((idx >> sh) & m), z = (int)(idx >> (2 * sh));
   uint32_t e[8];
-  for (int k = 0; k < 8; ++k) {
-    int xx = (x + (k & 1)) & m, yy = (y + ((k >> 1) & 1)) & m, zz = (z + (k >> 2)) & m;
-    float v[4];
-    texel4(src, fmt, (((size_t)zz << sh) + yy) * N + xx, v);
-    e[k] = pack_h2(v[0], (v[1] * 0.625f + v[2] * 0.25f) + v[3] * 0.125f);
-  }
+  for (int k = 0; k < 8; ++k) e[k] = lf_texel_h2(src, fmt, sh, (x + (k & 1)) & m, (y + ((k >> 1) & 1)) & m, (z + (k >> 2)) & m);
   out[2 * idx] = make_uint4(e[0], e[1], e[2], e[3]);
   out[2 * idx + 1] = make_uint4(e[4], e[5], e[6], e[7]);
+}
+
+// LF -> the same half2 texels in 4^3 bricks (N >= 8): brick (bx, by, bz)
+// of N/4 per axis at ((bz N/4 + by) N/4 + bx) * 64 texels, texel (x, y, z)
+// of a brick at x + 4 y + 16 z.  One lane per texel of the output.
+__global__ __launch_bounds__(256) void pack_lf_brick_kernel(const void* src, int fmt, int sh, uint32_t* out) {
+  const int N = 1 << sh, nbsh = sh - 2, nbm = (N >> 2) - 1;
+  const size_t idx = (size_t)blockIdx.x * 256 + threadIdx.x;
+  if (idx >= (size_t)N * N * N) return;
+  const size_t brick = idx >> 6;
+  const int t = (int)(idx & 63);
+  const int bx = (int)(brick & nbm), by = (int)((brick >> nbsh) & nbm), bz = (int)(brick >> (2 * nbsh));
+  out[idx] = lf_texel_h2(src, fmt, sh, 4 * bx + (t & 3), 4 * by + ((t >> 2) & 3), 4 * bz + (t >> 4));
 }
 
 // HF -> .625r + .25g + .125b at the 2x2x2 footprint
@@ -1072,11 +1270,18 @@ __global__ __launch_bounds__(256) void pack_map_kernel(const uint32_t* src, int 
 }  // namespace fast
 }  // namespace mtc
 
+// lf_brick: the bricked LF copy of MTC_FLAG_LDS_BRICKS contexts (N^3
+// half2 texels, lf_dim >= 8), or null
 hipError_t pack_fast_resources(const VolumeViews& vv, uint4* lf, uint4* hf, uint4* curl, uint4* weather,
-                               hipStream_t s) {
+                               uint32_t* lf_brick, hipStream_t s) {
   size_t n = (size_t)vv.lf_dim * vv.lf_dim * vv.lf_dim;
   hipLaunchKernelGGL(mtc::fast::pack_lf_kernel, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, vv.lf, vv.format,
                      vv.lf_shift, lf);
+  if (lf_brick) {
+    if (vv.lf_dim < 8) return hipErrorInvalidValue;
+    hipLaunchKernelGGL(mtc::fast::pack_lf_brick_kernel, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, vv.lf,
+                       vv.format, vv.lf_shift, lf_brick);
+  }
   n = (size_t)vv.hf_dim * vv.hf_dim * vv.hf_dim;
   hipLaunchKernelGGL(mtc::fast::pack_hf_kernel, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, vv.hf, vv.format,
                      vv.hf_shift, hf);
@@ -1089,7 +1294,7 @@ hipError_t pack_fast_resources(const VolumeViews& vv, uint4* lf, uint4* hf, uint
   return hipGetLastError();
 }
 
-template <int COV, bool F16, bool FP64, int G>
+template <int COV, bool F16, bool FP64, int G, bool BRICK = false>
 static void launch_v(hipStream_t s, const FrameConsts& fc, const mtc::fast::FastViews& fv, const OutputViews& ov,
                      int full, bool with_reproj, const KernelEvents& ke) {
   const int rays_x = full ? fc.width : (fc.width + 3) / 4;
@@ -1099,11 +1304,11 @@ static void launch_v(hipStream_t s, const FrameConsts& fc, const mtc::fast::Fast
   const int strips = with_reproj ? ((fc.width + 63) / 64) * ((fc.height + 3) / 4) : 0;
   dim3 grid((unsigned)(n_tiles + strips)), block(64);
   if (ov.counters)
-    launch_timed(mtc::fast::march_kernel<COV, F16, FP64, true, G>, grid, block, s, ke, fc, fv, ov, full, tiles_x,
-                 n_tiles);
+    launch_timed(mtc::fast::march_kernel<COV, F16, FP64, true, G, BRICK>, grid, block, s, ke, fc, fv, ov, full,
+                 tiles_x, n_tiles);
   else
-    launch_timed(mtc::fast::march_kernel<COV, F16, FP64, false, G>, grid, block, s, ke, fc, fv, ov, full, tiles_x,
-                 n_tiles);
+    launch_timed(mtc::fast::march_kernel<COV, F16, FP64, false, G, BRICK>, grid, block, s, ke, fc, fv, ov, full,
+                 tiles_x, n_tiles);
 }
 
 template <bool F16, bool FP64, int G>
@@ -1128,12 +1333,27 @@ static bool launch_g(int lanes_per_ray, hipStream_t s, const FrameConsts& fc, co
 // variant 0 = fast (FP32 filtering), 1 = precision (packed FP16
 // filtering); lanes_per_ray in {1, 2, 4, 8}; with_reproj (4x4 frames
 // only): the launch also carries the reprojection of the other pixels
-// (ov.prev -> ov.color)
+// (ov.prev -> ov.color).  lf_brick (lanes_per_ray 8 only): the bricked LF
+// copy, and the primary LF samples go through the waves' LDS boxes.
 hipError_t launch_march_fast(const FrameConsts& fc, const uint4* lf, const uint4* hf, const uint4* curl,
                              const uint4* weather, const VolumeViews& vv, const OutputViews& ov, int full, int variant,
-                             int lanes_per_ray, bool with_reproj, hipStream_t stream, const KernelEvents& ke) {
+                             int lanes_per_ray, bool with_reproj, hipStream_t stream, const KernelEvents& ke,
+                             const uint32_t* lf_brick) {
   if (with_reproj && (full || !ov.prev)) return hipErrorInvalidValue;
   mtc::fast::FastViews fv{lf, hf, curl, weather, vv.lf_shift, vv.hf_shift, vv.curl_shift, vv.weather_shift};
+  if (lf_brick) {
+    if (lanes_per_ray != 8 || vv.lf_dim < 8) return hipErrorInvalidValue;
+    fv.lf_brick = (const uint4*)lf_brick;
+    const bool w = fc.coverage_mode == MTC_COVERAGE_WEATHER, rp = with_reproj;
+    if (variant == 0) {
+      if (w) launch_v<MTC_COVERAGE_WEATHER, false, true, 8, true>(stream, fc, fv, ov, full, rp, ke);
+      else launch_v<MTC_COVERAGE_CONSTANT, false, true, 8, true>(stream, fc, fv, ov, full, rp, ke);
+    } else {
+      if (w) launch_v<MTC_COVERAGE_WEATHER, true, true, 8, true>(stream, fc, fv, ov, full, rp, ke);
+      else launch_v<MTC_COVERAGE_CONSTANT, true, true, 8, true>(stream, fc, fv, ov, full, rp, ke);
+    }
+    return hipGetLastError();
+  }
   // Both variants keep the shell quadratic in FP64: in FP32 its hit points
   // move by 0.3-1 m (C = dot(rO, rO) - 1 cancels), which shifts every t_k and
   // flips the int(t) jitter index of most steps (:631).

@@ -30,13 +30,14 @@ hipError_t launch_march_plain(const FrameConsts& fc, const VolumeViews& vv, cons
                               hipStream_t stream, const KernelEvents& ke);
 hipError_t launch_march_fast(const FrameConsts& fc, const uint4* lf, const uint4* hf, const uint4* curl,
                              const uint4* weather, const VolumeViews& vv, const OutputViews& ov, int full, int variant,
-                             int lanes_per_ray, bool with_reproj, hipStream_t stream, const KernelEvents& ke);
+                             int lanes_per_ray, bool with_reproj, hipStream_t stream, const KernelEvents& ke,
+                             const uint32_t* lf_brick);
 hipError_t launch_march_learned(const FrameConsts& fc, const uint4* lf, const uint4* hf, const uint4* curl,
                                 const uint4* weather, const VolumeViews& vv, const OutputViews& ov, int full,
                                 int lanes_per_ray, const LearnedLight& ll, bool with_reproj, const TileOrder& to,
                                 hipStream_t s, const KernelEvents& ke);
 hipError_t pack_fast_resources(const VolumeViews& vv, uint4* lf, uint4* hf, uint4* curl, uint4* weather,
-                               hipStream_t s);
+                               uint32_t* lf_brick, hipStream_t s);
 hipError_t launch_reproject(const FrameConsts& fc, const OutputViews& ov, int mode, hipStream_t stream,
                             const KernelEvents& ke = KernelEvents());
 hipError_t launch_noise(int kind, void* out, int N, int fmt, uint32_t seed, hipStream_t stream);
@@ -124,6 +125,7 @@ void release(mtc_ctx* c) {
   (void)hipFree(c->q_hf);
   (void)hipFree(c->q_curl);
   (void)hipFree(c->q_weather);
+  (void)hipFree(c->q_lf_brick);
   (void)hipFree(c->order);
   (void)hipFree(c->ray_fp);
   (void)hipFree(c->display);
@@ -169,11 +171,11 @@ int launch_march(mtc_ctx* c, const FrameConsts& fc, const OutputViews& ov, int f
     case MTC_KERNEL_PLAIN: e = launch_march_plain(fc, vv, ov, full, s, ke); break;
     case MTC_KERNEL_FAST:
       e = launch_march_fast(fc, c->q_lf, c->q_hf, c->q_curl, c->q_weather, vv, ov, full, 0, c->lanes_per_ray,
-                            with_reproj, s, ke);
+                            with_reproj, s, ke, c->q_lf_brick);
       break;
     case MTC_KERNEL_PRECISION:
       e = launch_march_fast(fc, c->q_lf, c->q_hf, c->q_curl, c->q_weather, vv, ov, full, 1, c->lanes_per_ray,
-                            with_reproj, s, ke);
+                            with_reproj, s, ke, c->q_lf_brick);
       break;
     case MTC_KERNEL_LEARNED: {
       LearnedLight ll;
@@ -246,7 +248,13 @@ int mtc_create(const mtc_desc* d, mtc_ctx** out) {
   if (d->lanes_per_ray != 0 && d->lanes_per_ray != 1 && d->lanes_per_ray != 2 && d->lanes_per_ray != 4 &&
       d->lanes_per_ray != 8)
     return mtc_fail(MTC_ERR_INVALID_ARG, "mtc_create: lanes_per_ray must be 0, 1, 2, 4 or 8");
-  if (d->flags & ~(MTC_FLAG_COUNTERS | MTC_FLAG_FULL_MARCH | MTC_FLAG_TIMING)) return mtc_fail(MTC_ERR_INVALID_ARG, "mtc_create: unknown flags");
+  if (d->flags & ~(MTC_FLAG_COUNTERS | MTC_FLAG_FULL_MARCH | MTC_FLAG_TIMING | MTC_FLAG_LDS_BRICKS))
+    return mtc_fail(MTC_ERR_INVALID_ARG, "mtc_create: unknown flags");
+  if ((d->flags & MTC_FLAG_LDS_BRICKS) &&
+      ((d->kernel != MTC_KERNEL_FAST && d->kernel != MTC_KERNEL_PRECISION) ||
+       (d->lanes_per_ray != 0 && d->lanes_per_ray != 8) || v.lf_dim < 8))
+    return mtc_fail(MTC_ERR_INVALID_ARG,
+                    "mtc_create: MTC_FLAG_LDS_BRICKS needs the fast or precision kernel, 8 lanes per ray and lf_dim >= 8");
   int ndev = 0;
   if (hipGetDeviceCount(&ndev) != hipSuccess || ndev <= 0)
     return mtc_fail(MTC_ERR_HIP, "mtc_create: no HIP device available");
@@ -277,6 +285,7 @@ int mtc_create(const mtc_desc* d, mtc_ctx** out) {
     if (e == hipSuccess) e = hipMalloc((void**)&c->q_hf, hf3 * sizeof(uint4));
     if (e == hipSuccess) e = hipMalloc((void**)&c->q_curl, (size_t)v.curl_dim * v.curl_dim * sizeof(uint4));
     if (e == hipSuccess) e = hipMalloc((void**)&c->q_weather, (size_t)v.weather_dim * v.weather_dim * sizeof(uint4));
+    if (e == hipSuccess && (d->flags & MTC_FLAG_LDS_BRICKS)) e = hipMalloc((void**)&c->q_lf_brick, lf3 * sizeof(uint32_t));
     if (d->kernel == MTC_KERNEL_LEARNED) {
       // tile order of the learned march, sized for the 4x4 frames' tiles of
       // 8 x 32/G rays (full frames have more tiles and go unordered)
@@ -319,7 +328,8 @@ int mtc_set_volumes(mtc_ctx* c, const void* lf, const void* hf, const void* curl
   HIP_TRY(hipMemcpyAsync(c->weather, weather, c->weather_bytes, hipMemcpyHostToDevice, c->stream),
           "mtc_set_volumes: weather");
   if (c->q_lf) {
-    HIP_TRY(pack_fast_resources(volume_views(c), c->q_lf, c->q_hf, c->q_curl, c->q_weather, c->stream),
+    HIP_TRY(pack_fast_resources(volume_views(c), c->q_lf, c->q_hf, c->q_curl, c->q_weather, c->q_lf_brick,
+                                c->stream),
             "mtc_set_volumes: footprint packing");
   }
   HIP_TRY(hipStreamSynchronize(c->stream), "mtc_set_volumes: sync");
@@ -337,7 +347,8 @@ int mtc_generate_volumes(mtc_ctx* c, const mtc_noise_desc* nd) {
   HIP_TRY(launch_noise(2, c->curl, v.curl_dim, MTC_FMT_UNORM8, nd->seed_volumes, c->stream), "noise curl");
   HIP_TRY(launch_noise(3, c->weather, v.weather_dim, MTC_FMT_UNORM8, nd->seed_weather, c->stream), "noise weather");
   if (c->q_lf) {
-    HIP_TRY(pack_fast_resources(volume_views(c), c->q_lf, c->q_hf, c->q_curl, c->q_weather, c->stream),
+    HIP_TRY(pack_fast_resources(volume_views(c), c->q_lf, c->q_hf, c->q_curl, c->q_weather, c->q_lf_brick,
+                                c->stream),
             "mtc_generate_volumes: footprint packing");
   }
   HIP_TRY(hipStreamSynchronize(c->stream), "mtc_generate_volumes: sync");
@@ -504,6 +515,8 @@ int mtc_get_counters(mtc_ctx* c, mtc_counters* out) {
   out->mlp_fallbacks = h[9];
   out->mlp_audits = h[10];
   out->mlp_audit_sq_err = h[11];
+  out->lf_brick_stages = h[12];
+  out->lf_brick_samples = h[13];
   return MTC_OK;
 }
 

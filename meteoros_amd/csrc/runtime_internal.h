@@ -191,6 +191,8 @@ struct mtc_ctx {
   uint4* q_hf = nullptr;
   uint4* q_curl = nullptr;
   uint4* q_weather = nullptr;
+  // MTC_FLAG_LDS_BRICKS: the LF's half2 (r, fbm) texels in 4^3 bricks
+  uint32_t* q_lf_brick = nullptr;
   // MTC_KERNEL_LEARNED: longest-first tile order, 3 slots of
   // order_slot_words(order_cap) words rotated by order_seq
   uint32_t* order = nullptr;

@@ -51,6 +51,7 @@ def _desc(**kw):
     d.volumes = A.MtcVolumeDesc(*kw.get("vol", (32, 16, 32, 64, A.MTC_FMT_UNORM8)))
     d.kernel = kw.get("kernel", A.MTC_KERNEL_PLAIN)
     d.flags = kw.get("flags", 0)
+    d.lanes_per_ray = kw.get("lanes", 0)
     return d
 
 
@@ -59,6 +60,8 @@ def _desc(**kw):
     dict(vol=(30, 16, 32, 64, 0)), dict(vol=(32, 16, 32, 64, 7)), dict(kernel=17), dict(flags=64),
     dict(vol=(2048, 16, 32, 64, 0)), dict(vol=(1024, 16, 32, 64, 0), kernel=A.MTC_KERNEL_FAST),
     dict(vol=(32, 1024, 32, 64, 1), kernel=A.MTC_KERNEL_PRECISION),
+    dict(flags=8), dict(flags=8, kernel=A.MTC_KERNEL_LEARNED), dict(flags=8, kernel=A.MTC_KERNEL_FAST, lanes=4),
+    dict(flags=8, kernel=A.MTC_KERNEL_PRECISION, vol=(4, 16, 32, 64, 0)),
 ])
 def test_create_rejects_invalid_descriptions(kw):
     ctx = C.c_void_p()
@@ -113,7 +116,7 @@ def test_missing_library_fails_loudly(monkeypatch, tmp_path):
 
 
 def test_learned_light_structs_and_null_handling():
-    assert C.sizeof(A.MtcCounters) == 12 * 8
+    assert C.sizeof(A.MtcCounters) == 14 * 8
     assert C.sizeof(A.MtcMlpTrainDesc) == 40 and A.MtcMlpTrainDesc.holdout.offset == 32
     assert C.sizeof(A.MtcMlpCalibration) == 156 and A.MtcMlpCalibration.mask.offset == 136
     lib = M.lib()
