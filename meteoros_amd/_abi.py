@@ -19,6 +19,7 @@ MTC_ERR_OUT_OF_MEMORY = 2
 MTC_ERR_HIP = 3
 MTC_ERR_UNSUPPORTED = 4
 MTC_ERR_STATE = 5
+MTC_GOD_RAYS_NOALLOC_MAX_ENTRIES = 2359296  # (width + 1) * (height + 1) limit of mtc_god_rays_noalloc
 
 MTC_STEPS_REFERENCE = 0
 MTC_STEPS_FIXED = 1
@@ -166,6 +167,8 @@ SIGNATURES = [
     ("mtc_load_image_rgba8", C.c_int, [C.c_char_p, P(C.c_int32), P(C.c_int32), C.c_void_p, C.c_size_t]),
     ("mtc_load_tga_slices", C.c_int, [C.c_char_p, C.c_char_p, C.c_int32, C.c_void_p, C.c_size_t]),
     ("mtc_god_rays", C.c_int, [C.c_void_p, P(MtcCamera), P(MtcSun), C.c_void_p]),
+    ("mtc_god_rays_noalloc", C.c_int, [C.c_int32, C.c_int32, P(MtcCamera), P(MtcSun), C.c_void_p, C.c_void_p,
+                                       C.c_void_p]),
     ("mtc_tone_map", C.c_int, [C.c_void_p, P(MtcTime), C.c_int32, C.c_void_p]),
     ("mtc_read_display", C.c_int, [C.c_void_p, C.c_void_p, C.c_size_t]),
     ("mtc_txaa", C.c_int, [C.c_void_p, P(MtcCamera), P(MtcCamera), P(MtcTime), C.c_void_p]),

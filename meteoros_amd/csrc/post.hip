@@ -29,6 +29,8 @@
 // sampler), Uncharted-2 curve with exposure 2.5 and white point 100, gamma
 // 1/2.2, plus 0.01 x WangHashNoise(x, y, uint(total time)); stored as
 // RGBA8 UNORM (the shader's rgba8 qualifier) or, for checking, as RGBA32F.
+#include <mutex>
+
 #include "device_common.h"
 
 namespace mtc {
@@ -276,6 +278,46 @@ __global__ __launch_bounds__(256) void txaa_kernel(TxaaConsts tc, const uint32_t
 
 }  // namespace post
 }  // namespace mtc
+
+// ---------------------------------------------------------------------------
+// Allocation-free god rays (mtc_god_rays_noalloc): the decoded-mask
+// footprint map lives in this file-scope device array instead of a buffer
+// the context allocates.  Each device holds its own copy, at its own
+// address, from the first load of this library until the process ends.
+// In host code `scratch` names a host-side stand-in, never device memory:
+// every launch takes the address hipGetSymbolAddress returns for the
+// current device (god_rays_scratch below).  Every call on one device shares
+// the array; the decode kernel rewrites every entry the blur reads, so
+// nothing has to start at zero, and calls that overlap on one device race.
+// ---------------------------------------------------------------------------
+namespace mtc {
+namespace post {
+static __device__ __align__(256) float4 scratch[MTC_GOD_RAYS_NOALLOC_MAX_ENTRIES];
+}  // namespace post
+}  // namespace mtc
+
+namespace {
+constexpr int kMaxScratchDevices = 64;
+std::mutex g_scratch_mu;
+float4* g_scratch_ptr[kMaxScratchDevices] = {};
+}  // namespace
+
+// Device address of `scratch` on `device`, which must be the current
+// device; looked up once per device and kept.
+hipError_t god_rays_scratch(int device, float4** out) {
+  *out = nullptr;
+  if (device < 0 || device >= kMaxScratchDevices) return hipErrorInvalidDevice;
+  std::lock_guard<std::mutex> lock(g_scratch_mu);
+  if (!g_scratch_ptr[device]) {
+    void* p = nullptr;
+    const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(mtc::post::scratch));
+    if (e != hipSuccess) return e;
+    if (!p) return hipErrorInvalidSymbol;
+    g_scratch_ptr[device] = (float4*)p;
+  }
+  *out = g_scratch_ptr[device];
+  return hipSuccess;
+}
 
 hipError_t launch_god_rays(const PostConsts& pc, const float4* mask, float4* footprint, float4* img, hipStream_t s) {
   const int W = pc.width, H = pc.height;

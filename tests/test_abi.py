@@ -129,3 +129,26 @@ def test_learned_light_structs_and_null_handling():
     assert lib.mtc_mlp_create(0, 0, C.byref(m)) == A.MTC_ERR_INVALID_ARG and not m
     if M.device_count() == 0:
         assert lib.mtc_mlp_create(0, 1024, C.byref(m)) == A.MTC_ERR_HIP and not m
+
+
+def test_god_rays_noalloc_rejects_bad_input_on_the_host():
+    """mtc_god_rays_noalloc validates everything before touching a device:
+    null pointers, non-positive sizes, sizes whose footprint map exceeds
+    the static scratch, and non-finite uniforms.  The pointers passed with
+    bad sizes are never dereferenced."""
+    lib = M.lib()
+    cam, sun = A.MtcCamera(), A.MtcSun()
+    for k in range(16):
+        cam.view[k] = cam.proj[k] = 1.0 if k % 5 == 0 else 0.0
+    dummy = C.c_void_p(256)
+    f = lib.mtc_god_rays_noalloc
+    assert f(64, 32, None, C.byref(sun), dummy, dummy, None) == A.MTC_ERR_INVALID_ARG
+    assert f(64, 32, C.byref(cam), C.byref(sun), None, dummy, None) == A.MTC_ERR_INVALID_ARG
+    assert f(0, 32, C.byref(cam), C.byref(sun), dummy, dummy, None) == A.MTC_ERR_INVALID_ARG
+    assert f(64, -1, C.byref(cam), C.byref(sun), dummy, dummy, None) == A.MTC_ERR_INVALID_ARG
+    limit = A.MTC_GOD_RAYS_NOALLOC_MAX_ENTRIES
+    assert f(2047, limit // 2048, C.byref(cam), C.byref(sun), dummy, dummy, None) == A.MTC_ERR_INVALID_ARG
+    assert b"MTC_GOD_RAYS_NOALLOC_MAX_ENTRIES" in lib.mtc_last_error()
+    assert f(3840, 2160, C.byref(cam), C.byref(sun), dummy, dummy, None) == A.MTC_ERR_INVALID_ARG
+    cam.eye[1] = float("nan")
+    assert f(64, 32, C.byref(cam), C.byref(sun), dummy, dummy, None) == A.MTC_ERR_INVALID_ARG
