@@ -368,20 +368,27 @@ enum {
  * the latest result image, in place.  Uses camera view/proj/eye and
  * sun->light_color.  A sun behind the camera leaves the image unchanged. */
 int mtc_god_rays(mtc_ctx* ctx, const mtc_camera* cam, const mtc_sun* sun, void* stream);
-/* Largest (width + 1) * (height + 1) that mtc_god_rays_noalloc accepts:
- * the entries of its decoded-mask footprint map, 16 bytes each (36 MiB),
- * enough for 1920x1200 or 2048x1080. */
+/* Sizes that mtc_god_rays_noalloc accepts: (width + 1) * (height + 1) up
+ * to MTC_GOD_RAYS_NOALLOC_MAX_ENTRIES (e.g. 1920x1200 or 2048x1080),
+ * height up to MTC_GOD_RAYS_NOALLOC_MAX_HEIGHT and width up to
+ * MTC_GOD_RAYS_NOALLOC_MAX_WIDTH.  Within them its scratch (the decoded
+ * mask, padded, and the blur's table of 1 700 bytes per pixel row) fits in
+ * MTC_GOD_RAYS_NOALLOC_SCRATCH_BYTES (18 MiB). */
 #define MTC_GOD_RAYS_NOALLOC_MAX_ENTRIES 2359296
+#define MTC_GOD_RAYS_NOALLOC_MAX_HEIGHT 4320
+#define MTC_GOD_RAYS_NOALLOC_MAX_WIDTH 16384
+#define MTC_GOD_RAYS_NOALLOC_SCRATCH_BYTES 18874368
 /* No counterpart in the reference: an entry point that allocates nothing,
  * for callers that cannot allocate device memory (code under HIP graph
  * capture, tight serving loops, harnesses that forbid hipMalloc).  The
  * god-ray pass of mtc_god_rays, without a context, on caller-owned device
  * memory of the current device: `mask` is the width*height RGBA32F god-ray
  * mask and `image` the width*height RGBA32F result, updated in place.  The
- * pass's scratch (the decoded-mask footprint map) is a static device array
- * of MTC_GOD_RAYS_NOALLOC_MAX_ENTRIES entries that each device holds from
- * the library's first load until the process ends; larger sizes are
- * rejected (MTC_ERR_INVALID_ARG) before anything is enqueued.  Enqueued on
+ * pass's scratch (the decoded mask and the blur's table) is a static
+ * device array of MTC_GOD_RAYS_NOALLOC_SCRATCH_BYTES that each device holds
+ * from the library's first load until the process ends; sizes beyond the
+ * limits above are rejected (MTC_ERR_INVALID_ARG) before anything is
+ * enqueued.  Enqueued on
  * `stream` (NULL = the null stream).  All calls on one device share that
  * array, so calls that overlap on one device race.  The first call on a
  * device looks up the array's address; make it outside graph capture. */

@@ -20,6 +20,9 @@ MTC_ERR_HIP = 3
 MTC_ERR_UNSUPPORTED = 4
 MTC_ERR_STATE = 5
 MTC_GOD_RAYS_NOALLOC_MAX_ENTRIES = 2359296  # (width + 1) * (height + 1) limit of mtc_god_rays_noalloc
+MTC_GOD_RAYS_NOALLOC_MAX_HEIGHT = 4320  # height limit of mtc_god_rays_noalloc
+MTC_GOD_RAYS_NOALLOC_MAX_WIDTH = 16384  # width limit of mtc_god_rays_noalloc
+MTC_GOD_RAYS_NOALLOC_SCRATCH_BYTES = 18874368  # its static scratch, bytes per device
 
 MTC_STEPS_REFERENCE = 0
 MTC_STEPS_FIXED = 1

@@ -13,10 +13,11 @@
 // God rays.  100 bilinear samples of the decoded god-ray mask along the line
 // from the pixel to the sun's screen position.  The mask is RGBA (the
 // EncodeFloatRGBA of cloudRayMarch.comp:105-112); decoding is linear, so it
-// is decoded once per frame into a footprint-packed map (each entry holds
-// the 2x2 texels a bilinear tap reads, border texels included) and each of
-// the 100 samples is one 16-byte load instead of four.  Sampler: linear,
-// CLAMP_TO_BORDER with opaque black (Texture2D.cpp:75, Image.cpp:305-345).
+// is decoded once per frame into a float map (border texels included), and
+// each workgroup stages the part of that map its taps read at a sample into
+// LDS, so the 100 taps per pixel read LDS rather than the texture path.
+// Sampler: linear, CLAMP_TO_BORDER with opaque black (Texture2D.cpp:75,
+// Image.cpp:305-345).
 //
 // TXAA.  Re-projects each pixel into the previous TXAA output through the
 // inner atmosphere shell of the TXAA shader (R + 7.5 km), clips the
@@ -47,55 +48,188 @@ __device__ __forceinline__ float decode(float4 c) {
   return ((c.x * 1.0f + c.y * d1) + c.z * d2) + c.w * d3;
 }
 
-// Entry (i, j), i in [-1, W-1], j in [-1, H-1], at ((j + 1) * (W + 1) + i + 1):
-// decoded texels (i, j), (i+1, j), (i, j+1), (i+1, j+1).
-__global__ __launch_bounds__(256) void decode_mask_kernel(const float4* __restrict__ mask, int W, int H,
-                                                          float4* __restrict__ fp) {
-  const int i = blockIdx.x * 64 + (threadIdx.x & 63) - 1;
-  const int j = blockIdx.y * 4 + (threadIdx.x >> 6) - 1;
-  if (i >= W || j >= H) return;
-  const float a = decode(texel_or_border(mask, W, H, i, j));
-  const float b = decode(texel_or_border(mask, W, H, i + 1, j));
-  const float c = decode(texel_or_border(mask, W, H, i, j + 1));
-  const float d = decode(texel_or_border(mask, W, H, i + 1, j + 1));
-  fp[(size_t)(j + 1) * (W + 1) + (i + 1)] = make_float4(a, b, c, d);
+// The god-ray set-up launch: the blur's table, then the decoded mask.
+//
+// Table.  All samples of one pixel row share their vertical position: v
+// starts at (py + 0.5) / H and steps by the row's own delta_uv.y, in the
+// shader's FP32 order, so sample i of every pixel in the row reads the same
+// texel rows with the same vertical weights.  One wave per row group of
+// kGodRayGroupRows rows: lane = 16 * chunk + k walks row k's recurrence and
+// writes samples 25 * chunk .. 25 * chunk + 24 (re-running the recurrence's
+// subtractions up to its chunk, the same FP32 operations), with the group's
+// lowest texel row (a 16-lane minimum) in the sample's header entry.
+//
+// Map.  One lane per map texel: dot(rgba, bitDec) of the mask texel, or of
+// the sampler's opaque black border outside the image.  Decoding is linear,
+// so decoding before filtering equals the shader's filtering before
+// decoding up to FP32 rounding.
+__global__ __launch_bounds__(256) void god_rays_setup_kernel(PostConsts pc, const float4* __restrict__ mask,
+                                                             int table_blocks, float* __restrict__ map,
+                                                             GodRayEntry* __restrict__ tab) {
+  const int W = pc.width, H = pc.height;
+  if ((int)blockIdx.x < table_blocks) {
+    const int g = (int)blockIdx.x * 4 + (int)(threadIdx.x >> 6);
+    const int lane = threadIdx.x & 63;
+    if (g >= (H + kGodRayGroupRows - 1) / kGodRayGroupRows) return;
+    const int k = lane & 15, chunk = lane >> 4;
+    const int py = min(g * kGodRayGroupRows + k, H - 1);
+    const float Hf = (float)H;
+    float v = ((float)py + 0.5f) / Hf;
+    // delta_uv.y = ((uv.y - sunPos_ss.y) / numSamples) * ATMOSPHERE_DENSITY
+    const float dv = ((v - pc.sun_ss[1]) / (float)kGodRaySamples) * 1.0f;
+    for (int s = 0; s < 25 * chunk; ++s) v -= dv;
+    GodRayEntry* out = tab + (size_t)g * kGodRaySamples * kGodRayTableStride;
+    for (int j = 0; j < 25; ++j) {
+      const int i = 25 * chunk + j;
+      const float y = v * Hf - 0.5f;
+      const float fy = floorf(y);
+      const float wy = y - fy;
+      // the walk stays inside the segment pixel -> sun, so floor lies in [-1, H - 1]
+      const int iy = min(max((int)fy, -1), H - 1);
+      int lo = iy;
+      for (int m = 1; m < 16; m <<= 1) lo = min(lo, __shfl_xor(lo, m, 16));
+      out[(size_t)i * kGodRayTableStride + 1 + k] =
+          GodRayEntry{(uint32_t)((iy - lo) * kGodRayStageCols * (int)sizeof(float)), 1.0f - wy, wy, 0u};
+      if (k == 0) out[(size_t)i * kGodRayTableStride] = GodRayEntry{(uint32_t)(lo + 1), 0.0f, 0.0f, 0u};
+      v -= dv;
+    }
+    return;
+  }
+  const int P = god_rays_map_pitch(W);
+  const long long n = ((long long)blockIdx.x - table_blocks) * 256 + threadIdx.x;
+  if (n >= (long long)god_rays_map_rows(H) * P) return;
+  const int row = (int)(n / P), col = (int)(n - (long long)row * P);
+  map[n] = decode(texel_or_border(mask, W, H, col - 4, row - 1));
 }
 
-__global__ __launch_bounds__(256) void god_rays_kernel(PostConsts pc, const float4* __restrict__ fp,
-                                                       float4* __restrict__ img) {
+// The radial blur.  Workgroup = 64 columns x kGodRayGroupRows rows, four
+// waves of four rows per lane.  Pixels of one column start at the same u
+// and step by the same delta_uv.x, so a lane computes each sample's
+// horizontal position once for its four pixels, and at sample i the taps
+// of the whole workgroup lie in one region of the decoded map: at most
+// kGodRayStageRows rows (from the table's header) x kGodRayStageCols
+// columns (from the first lane's column, aligned down to 4).  The workgroup
+// stages that region into LDS with 16-byte loads (324 of them, two per
+// lane at most) and every tap reads its four texels from LDS.  Staging runs
+// two samples ahead in registers and one ahead in the second LDS buffer,
+// the table entries one sample ahead in scalar registers; one barrier per
+// sample.  Per tap: two ds_read2_b32 and six FMA-class operations, the
+// same arithmetic as a per-pixel bilinear tap.
+__global__ __launch_bounds__(256) void god_rays_kernel(PostConsts pc, const float* __restrict__ map,
+                                                       const GodRayEntry* __restrict__ tab, float4* __restrict__ img) {
+  constexpr int REGION = kGodRayStageRows * kGodRayStageCols;
+  constexpr int BLOCKS = kGodRayStageCols / 4;
+  constexpr int SLOTS = kGodRayStageRows * BLOCKS;
+  static_assert(SLOTS > 256 && SLOTS <= 512, "two staging loads per lane");
+  __shared__ __attribute__((aligned(16))) float stage[2 * REGION];
+  typedef float f4v __attribute__((ext_vector_type(4)));
+  typedef const __attribute__((address_space(1))) char gchar;
   const int W = pc.width, H = pc.height;
-  const int px = blockIdx.x * 64 + (threadIdx.x & 63);
-  const int py = blockIdx.y * 4 + (threadIdx.x >> 6);
-  if (px >= W || py >= H) return;
-  float u = ((float)px + 0.5f) / (float)W, v = ((float)py + 0.5f) / (float)H;
-  // delta_uv = ((uv - sunPos_ss) / numSamples) * ATMOSPHERE_DENSITY
-  const float du = ((u - pc.sun_ss[0]) / 100.0f) * 1.0f, dv = ((v - pc.sun_ss[1]) / 100.0f) * 1.0f;
-  const float Wf = (float)W, Hf = (float)H;
-  float ar = 0.0f, ag = 0.0f, ab = 0.0f;
-  for (int i = 0; i < 100; ++i) {
-    const float x = u * Wf - 0.5f, y = v * Hf - 0.5f;
-    const float fx = floorf(x), fy = floorf(y);
-    const float wx = x - fx, wy = y - fy;
-    // the walk stays inside the segment pixel -> sun (both in [0, 1]^2), so
-    // floor lies in [-1, dim - 1]
-    const int ix = min(max((int)fx, -1), W - 1), iy = min(max((int)fy, -1), H - 1);
-    const float4 e = fp[(size_t)(iy + 1) * (W + 1) + (ix + 1)];
-    const float a = lerpf(lerpf(e.x, e.y, wx), lerpf(e.z, e.w, wx), wy);
-    // godRayLoop: (rgb * a) * (decay * WEIGHT_FOR_SAMPLE), decay = 1
-    ar += (pc.light[0] * a) * 0.001f;
-    ag += (pc.light[1] * a) * 0.001f;
-    ab += (pc.light[2] * a) * 0.001f;
+  const int P = god_rays_map_pitch(W);
+  const int tid = threadIdx.x, lane = tid & 63;
+  const int w = __builtin_amdgcn_readfirstlane(tid >> 6);
+  const int px = blockIdx.x * 64 + lane;
+  const int g = blockIdx.y;
+  // lanes right of the image walk the last column's line and store nothing
+  float u = ((float)min(px, W - 1) + 0.5f) / (float)W;
+  // delta_uv.x = ((uv.x - sunPos_ss.x) / numSamples) * ATMOSPHERE_DENSITY
+  const float du = ((u - pc.sun_ss[0]) / (float)kGodRaySamples) * 1.0f;
+  const float Wf = (float)W;
+  const GodRayEntry* __restrict__ gt = tab + (size_t)g * kGodRaySamples * kGodRayTableStride;
+  // this lane's staging slots (row, 4-column block) in the map and in LDS
+  const int t0 = tid, t1 = tid + 256;
+  const bool has1 = t1 < SLOTS;
+  const uint32_t goff0 = (uint32_t)((t0 / BLOCKS) * P + 4 * (t0 % BLOCKS)) * 4u;
+  const uint32_t goff1 = has1 ? (uint32_t)((t1 / BLOCKS) * P + 4 * (t1 % BLOCKS)) * 4u : 0u;
+  const uint32_t loff0 = (uint32_t)((t0 / BLOCKS) * kGodRayStageCols + 4 * (t0 % BLOCKS)) * 4u;
+  const uint32_t loff1 = has1 ? (uint32_t)((t1 / BLOCKS) * kGodRayStageCols + 4 * (t1 % BLOCKS)) * 4u : 0u;
+  // sample position: x = u * W - 0.5 (sampler), weight and column; the
+  // region's first column, 4-aligned, from the lowest lane's column
+  auto advance = [&](float& wx, int& ix, int& cs) {
+    const float x = u * Wf - 0.5f;
+    const float fx = floorf(x);
+    wx = x - fx;
+    // the walk stays between pixel and sun: floor lies in [-1, W - 1]
+    ix = min(max((int)fx, -1), W - 1);
+    cs = (__builtin_amdgcn_readfirstlane(ix) + 4) & ~3;
     u -= du;
-    v -= dv;
+  };
+  // the region's 16-byte blocks of the sample whose first map row is `row`
+  auto load_region = [&](uint32_t row, int cs, f4v& d0, f4v& d1) {
+    const uint64_t rb = (uint64_t)map + ((uint64_t)row * (uint64_t)P + (uint64_t)cs) * sizeof(float);
+    // the region's address is uniform: kept whole in scalar registers (each
+    // half through uint32_t, since readfirstlane returns int)
+    const uint32_t hi = (uint32_t)__builtin_amdgcn_readfirstlane((int)(uint32_t)(rb >> 32));
+    const uint32_t lo = (uint32_t)__builtin_amdgcn_readfirstlane((int)(uint32_t)rb);
+    gchar* b = (gchar*)(((uint64_t)hi << 32) | (uint64_t)lo);
+    d0 = *(const __attribute__((address_space(1))) f4v*)(b + goff0);
+    if (has1) d1 = *(const __attribute__((address_space(1))) f4v*)(b + goff1);
+  };
+  float wx_c, wx_n;
+  int ix_c, ix_n, cs_c, cs_n;
+  f4v d0, d1;
+  advance(wx_c, ix_c, cs_c);
+  load_region(gt[0].a, cs_c, d0, d1);
+  *(f4v*)((char*)stage + loff0) = d0;
+  if (has1) *(f4v*)((char*)stage + loff1) = d1;
+  advance(wx_n, ix_n, cs_n);
+  load_region(gt[kGodRayTableStride].a, cs_n, d0, d1);
+  GodRayEntry nx[4];
+#pragma unroll
+  for (int k = 0; k < 4; ++k) nx[k] = gt[1 + 4 * w + k];
+  uint32_t row2 = gt[2 * kGodRayTableStride].a;
+  __syncthreads();
+  float acc[4] = {0.0f, 0.0f, 0.0f, 0.0f};
+  for (int i = 0; i < kGodRaySamples; ++i) {
+    GodRayEntry cur[4];
+#pragma unroll
+    for (int k = 0; k < 4; ++k) cur[k] = nx[k];
+    const uint32_t row = row2;
+    if (i + 1 < kGodRaySamples) {
+#pragma unroll
+      for (int k = 0; k < 4; ++k) nx[k] = gt[(size_t)(i + 1) * kGodRayTableStride + 1 + 4 * w + k];
+    }
+    if (i + 3 < kGodRaySamples) row2 = gt[(size_t)(i + 3) * kGodRayTableStride].a;
+    // sample i from stage[i & 1]
+    const char* sb = (const char*)(stage + (i & 1) * REGION) + (uint32_t)(ix_c + 4 - cs_c) * 4u;
+#pragma unroll
+    for (int k = 0; k < 4; ++k) {
+      const float* q = (const float*)(sb + cur[k].a);
+      const float a0 = q[0], a1 = q[1], b0 = q[kGodRayStageCols], b1 = q[kGodRayStageCols + 1];
+      const float t = __fmaf_rn(wx_c, a1 - a0, a0), b = __fmaf_rn(wx_c, b1 - b0, b0);
+      // godRayLoop with decay 1: the mask value summed here, the light
+      // colour and WEIGHT_FOR_SAMPLE applied once below
+      acc[k] = __fmaf_rn(cur[k].w1, b, __fmaf_rn(cur[k].w0, t, acc[k]));
+    }
+    // sample i + 1's region (loaded one iteration ago) into the other buffer
+    if (i + 1 < kGodRaySamples) {
+      char* wb = (char*)(stage + ((i + 1) & 1) * REGION);
+      *(f4v*)(wb + loff0) = d0;
+      if (has1) *(f4v*)(wb + loff1) = d1;
+    }
+    wx_c = wx_n;
+    ix_c = ix_n;
+    cs_c = cs_n;
+    if (i + 2 < kGodRaySamples) {
+      advance(wx_n, ix_n, cs_n);
+      load_region(row, cs_n, d0, d1);
+    }
+    __syncthreads();
   }
-  const size_t o = (size_t)py * W + px;
-  float4 c = img[o];
-  // original + vec4(accum * EXPOSURE, 1) * blendFactor
-  c.x += (ar * 1.0f) * pc.blend;
-  c.y += (ag * 1.0f) * pc.blend;
-  c.z += (ab * 1.0f) * pc.blend;
-  c.w += 1.0f * pc.blend;
-  img[o] = c;
+  if (px >= W) return;
+#pragma unroll
+  for (int k = 0; k < 4; ++k) {
+    const int py = g * kGodRayGroupRows + 4 * w + k;
+    if (py >= H) break;
+    const size_t o = (size_t)py * W + px;
+    float4 c = img[o];
+    // original + vec4(accum * EXPOSURE, 1) * blendFactor, accum = light * sum * 0.001
+    c.x += (((pc.light[0] * acc[k]) * 0.001f) * 1.0f) * pc.blend;
+    c.y += (((pc.light[1] * acc[k]) * 0.001f) * 1.0f) * pc.blend;
+    c.z += (((pc.light[2] * acc[k]) * 0.001f) * 1.0f) * pc.blend;
+    c.w += 1.0f * pc.blend;
+    img[o] = c;
+  }
 }
 
 // ((x (A x + C B) + D E) / (x (A x + B) + D F)) - E / F
@@ -280,31 +414,32 @@ __global__ __launch_bounds__(256) void txaa_kernel(TxaaConsts tc, const uint32_t
 }  // namespace mtc
 
 // ---------------------------------------------------------------------------
-// Allocation-free god rays (mtc_god_rays_noalloc): the decoded-mask
-// footprint map lives in this file-scope device array instead of a buffer
-// the context allocates.  Each device holds its own copy, at its own
-// address, from the first load of this library until the process ends.
-// In host code `scratch` names a host-side stand-in, never device memory:
-// every launch takes the address hipGetSymbolAddress returns for the
-// current device (god_rays_scratch below).  Every call on one device shares
-// the array; the decode kernel rewrites every entry the blur reads, so
-// nothing has to start at zero, and calls that overlap on one device race.
+// Allocation-free god rays (mtc_god_rays_noalloc): the pass's scratch (the
+// decoded mask and the blur's table) lives in this
+// file-scope device array instead of a buffer the context allocates.  Each
+// device holds its own copy, at its own address, from the first load of
+// this library until the process ends.  In host code `scratch` names a
+// host-side stand-in, never device memory: every launch takes the address
+// hipGetSymbolAddress returns for the current device (god_rays_scratch
+// below).  Every call on one device shares the array; the set-up launch
+// rewrites every byte the blur reads, so nothing has to start at zero, and
+// calls that overlap on one device race.
 // ---------------------------------------------------------------------------
 namespace mtc {
 namespace post {
-static __device__ __align__(256) float4 scratch[MTC_GOD_RAYS_NOALLOC_MAX_ENTRIES];
+static __device__ __align__(256) float4 scratch[MTC_GOD_RAYS_NOALLOC_SCRATCH_BYTES / sizeof(float4)];
 }  // namespace post
 }  // namespace mtc
 
 namespace {
 constexpr int kMaxScratchDevices = 64;
 std::mutex g_scratch_mu;
-float4* g_scratch_ptr[kMaxScratchDevices] = {};
+void* g_scratch_ptr[kMaxScratchDevices] = {};
 }  // namespace
 
 // Device address of `scratch` on `device`, which must be the current
 // device; looked up once per device and kept.
-hipError_t god_rays_scratch(int device, float4** out) {
+hipError_t god_rays_scratch(int device, void** out) {
   *out = nullptr;
   if (device < 0 || device >= kMaxScratchDevices) return hipErrorInvalidDevice;
   std::lock_guard<std::mutex> lock(g_scratch_mu);
@@ -313,17 +448,25 @@ hipError_t god_rays_scratch(int device, float4** out) {
     const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(mtc::post::scratch));
     if (e != hipSuccess) return e;
     if (!p) return hipErrorInvalidSymbol;
-    g_scratch_ptr[device] = (float4*)p;
+    g_scratch_ptr[device] = p;
   }
   *out = g_scratch_ptr[device];
   return hipSuccess;
 }
 
-hipError_t launch_god_rays(const PostConsts& pc, const float4* mask, float4* footprint, float4* img, hipStream_t s) {
+// `scratch` holds god_rays_scratch_bytes(W, H) bytes: the decoded map,
+// then the blur's table.
+hipError_t launch_god_rays(const PostConsts& pc, const float4* mask, void* scratch, float4* img, hipStream_t s) {
   const int W = pc.width, H = pc.height;
-  dim3 gd((W + 1 + 63) / 64, (H + 1 + 3) / 4), gp((W + 63) / 64, (H + 3) / 4), block(256);
-  hipLaunchKernelGGL(mtc::post::decode_mask_kernel, gd, block, 0, s, mask, W, H, footprint);
-  hipLaunchKernelGGL(mtc::post::god_rays_kernel, gp, block, 0, s, pc, (const float4*)footprint, img);
+  float* map = (float*)scratch;
+  GodRayEntry* tab = (GodRayEntry*)((char*)scratch + god_rays_map_span(W, H));
+  const int groups = (H + kGodRayGroupRows - 1) / kGodRayGroupRows;
+  const int table_blocks = (groups + 3) / 4;
+  const long long map_blocks = ((long long)god_rays_map_rows(H) * god_rays_map_pitch(W) + 255) / 256;
+  hipLaunchKernelGGL(mtc::post::god_rays_setup_kernel, dim3((unsigned)(table_blocks + map_blocks)), dim3(256), 0, s, pc,
+                     mask, table_blocks, map, tab);
+  hipLaunchKernelGGL(mtc::post::god_rays_kernel, dim3((W + 63) / 64, groups), dim3(256), 0, s, pc, (const float*)map,
+                     (const GodRayEntry*)tab, img);
   return hipGetLastError();
 }
 

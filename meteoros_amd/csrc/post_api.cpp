@@ -15,8 +15,8 @@
 
 #include "runtime_internal.h"
 
-hipError_t launch_god_rays(const PostConsts& pc, const float4* mask, float4* footprint, float4* img, hipStream_t s);
-hipError_t god_rays_scratch(int device, float4** out);
+hipError_t launch_god_rays(const PostConsts& pc, const float4* mask, void* scratch, float4* img, hipStream_t s);
+hipError_t god_rays_scratch(int device, void** out);
 hipError_t launch_tone_map(const float4* img, int W, int H, uint32_t seed_s, float whitemap, int f32, void* out,
                            hipStream_t s);
 hipError_t launch_txaa(const TxaaConsts& tc, const uint32_t* cur, const uint32_t* prev, uint32_t* out, hipStream_t s);
@@ -156,10 +156,10 @@ int mtc_god_rays(mtc_ctx* c, const mtc_camera* cam, const mtc_sun* sun, void* st
   const int W = c->desc.width, H = c->desc.height;
   PostConsts pc;
   if (!god_ray_consts(cam, sun, W, H, &pc)) return MTC_OK;  // sun behind the camera: nothing to add
-  if (!c->ray_fp)
-    POST_OK(hipMalloc((void**)&c->ray_fp, (size_t)(W + 1) * (H + 1) * sizeof(float4)), "mtc_god_rays: footprint map");
+  if (!c->ray_scratch)
+    POST_OK(hipMalloc(&c->ray_scratch, god_rays_scratch_bytes(W, H)), "mtc_god_rays: decoded mask and blur table");
   hipStream_t s = stream ? (hipStream_t)stream : c->stream;
-  POST_OK(launch_god_rays(pc, c->mask, c->ray_fp, c->img[c->latest], s), "mtc_god_rays: launch");
+  POST_OK(launch_god_rays(pc, c->mask, c->ray_scratch, c->img[c->latest], s), "mtc_god_rays: launch");
   return MTC_OK;
 }
 
@@ -172,6 +172,17 @@ int mtc_god_rays_noalloc(int32_t width, int32_t height, const mtc_camera* cam, c
     return mtc_fail(MTC_ERR_INVALID_ARG,
                     "mtc_god_rays_noalloc: (width + 1) * (height + 1) exceeds MTC_GOD_RAYS_NOALLOC_MAX_ENTRIES (" +
                         std::to_string(MTC_GOD_RAYS_NOALLOC_MAX_ENTRIES) + ")");
+  if (height > MTC_GOD_RAYS_NOALLOC_MAX_HEIGHT)
+    return mtc_fail(MTC_ERR_INVALID_ARG, "mtc_god_rays_noalloc: height exceeds MTC_GOD_RAYS_NOALLOC_MAX_HEIGHT (" +
+                                             std::to_string(MTC_GOD_RAYS_NOALLOC_MAX_HEIGHT) + ")");
+  if (width > MTC_GOD_RAYS_NOALLOC_MAX_WIDTH)
+    return mtc_fail(MTC_ERR_INVALID_ARG, "mtc_god_rays_noalloc: width exceeds MTC_GOD_RAYS_NOALLOC_MAX_WIDTH (" +
+                                             std::to_string(MTC_GOD_RAYS_NOALLOC_MAX_WIDTH) + ")");
+  // the three limits above bound the scratch the pass needs by the static
+  // array's size; checked here so that a change of layout cannot overrun it
+  if (god_rays_scratch_bytes(width, height) > (size_t)MTC_GOD_RAYS_NOALLOC_SCRATCH_BYTES)
+    return mtc_fail(MTC_ERR_INVALID_ARG, "mtc_god_rays_noalloc: " + std::to_string(width) + "x" +
+                                             std::to_string(height) + " needs more than MTC_GOD_RAYS_NOALLOC_SCRATCH_BYTES");
   if (!finite_all(cam->view, 16) || !finite_all(cam->proj, 16) || !finite_all(cam->eye, 4) ||
       !finite_all(sun->light_color, 4))
     return mtc_fail(MTC_ERR_INVALID_ARG, "mtc_god_rays_noalloc: non-finite camera or light colour");
@@ -179,9 +190,9 @@ int mtc_god_rays_noalloc(int32_t width, int32_t height, const mtc_camera* cam, c
   if (!god_ray_consts(cam, sun, width, height, &pc)) return MTC_OK;  // sun behind the camera: nothing to add
   int device = -1;
   POST_OK(hipGetDevice(&device), "mtc_god_rays_noalloc: hipGetDevice");
-  float4* fp = nullptr;
-  POST_OK(god_rays_scratch(device, &fp), "mtc_god_rays_noalloc: scratch address");
-  POST_OK(launch_god_rays(pc, (const float4*)mask, fp, (float4*)image, (hipStream_t)stream),
+  void* scratch = nullptr;
+  POST_OK(god_rays_scratch(device, &scratch), "mtc_god_rays_noalloc: scratch address");
+  POST_OK(launch_god_rays(pc, (const float4*)mask, scratch, (float4*)image, (hipStream_t)stream),
           "mtc_god_rays_noalloc: launch");
   return MTC_OK;
 }

@@ -127,7 +127,7 @@ void release(mtc_ctx* c) {
   (void)hipFree(c->q_weather);
   (void)hipFree(c->q_lf_brick);
   (void)hipFree(c->order);
-  (void)hipFree(c->ray_fp);
+  (void)hipFree(c->ray_scratch);
   (void)hipFree(c->display);
   (void)hipFree(c->txaa[0]);
   (void)hipFree(c->txaa[1]);

@@ -149,6 +149,44 @@ struct PostConsts {
   int32_t width, height;
 };
 
+// God-ray blur geometry.  A workgroup (four waves) blurs 64 columns x
+// kGodRayGroupRows pixel rows; each wave holds four of the rows per lane.
+// At every sample the texels the workgroup's taps read lie in at most
+// kGodRayStageRows rows x kGodRayStageCols columns of the decoded mask,
+// which the workgroup stages into LDS.
+constexpr int kGodRaySamples = 100;                    // MAX_SAMPLES, postProcess_GodRays.frag:25
+constexpr int kGodRayGroupRows = 16;
+constexpr int kGodRayStageRows = kGodRayGroupRows + 2;  // 16 rows at spacing <= 1 span <= 17 texel rows + 1
+constexpr int kGodRayStageCols = 72;                   // 64 lanes at spacing <= 1 + 4-aligned start + 1
+constexpr int kGodRayTableStride = kGodRayGroupRows + 1;  // entries per (row group, sample)
+
+// One entry of the blur's table, per (row group, sample): entry 0 holds in
+// `a` the first decoded-map row staged for the sample; entry 1 + k, for
+// pixel row k of the group, the byte offset in the staged region of the
+// texel row its tap reads, with the vertical weights w0 = 1 - wy, w1 = wy.
+struct GodRayEntry {
+  uint32_t a;
+  float w0, w1;
+  uint32_t pad;
+};
+
+// The decoded mask: map row r + 1, column c + 4 hold texel (c, r) for
+// r in [-1, H], c in [-4, W + 71] (border value outside the image); rows
+// and columns beyond the image let every staged region be read whole.
+__host__ __device__ inline int god_rays_map_pitch(int W) { return (W + 76 + 3) / 4 * 4; }
+__host__ __device__ inline int god_rays_map_rows(int H) { return H + 19; }
+inline size_t god_rays_map_bytes(int W, int H) {
+  return (size_t)god_rays_map_rows(H) * (size_t)god_rays_map_pitch(W) * sizeof(float);
+}
+inline size_t god_rays_table_bytes(int H) {
+  return (size_t)((H + kGodRayGroupRows - 1) / kGodRayGroupRows) * kGodRaySamples * kGodRayTableStride *
+         sizeof(GodRayEntry);
+}
+// Bytes of the pass's scratch at W x H: the map (256-byte aligned size),
+// then the table.
+inline size_t god_rays_map_span(int W, int H) { return (god_rays_map_bytes(W, H) + 255) / 256 * 256; }
+inline size_t god_rays_scratch_bytes(int W, int H) { return god_rays_map_span(W, H) + god_rays_table_bytes(H); }
+
 // Per-frame constants of the TXAA pass (postProcess_TXAA.frag:228-246),
 // folded on the host in FP32 with the shader's operation order.
 struct TxaaConsts {
@@ -208,9 +246,10 @@ struct mtc_ctx {
   mtc_timings totals = {0, 0.0, 0.0};
   int32_t timing_stride = 1;   // events on one frame in timing_stride
   uint64_t frames_seen = 0;    // frames since creation (for the stride)
-  // post-process: decoded god-ray mask footprints ((W+1) x (H+1) float4)
-  // and the display image (RGBA8 or RGBA32F), allocated on first use
-  float4* ray_fp = nullptr;
+  // post-process: the god-ray pass's scratch (decoded mask and blur table,
+  // god_rays_scratch_bytes) and the display image (RGBA8 or RGBA32F),
+  // allocated on first use
+  void* ray_scratch = nullptr;
   void* display = nullptr;
   int display_format = -1;
   // TXAA: RGBA8 history / output ping-pong, txaa_latest = the last output

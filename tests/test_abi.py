@@ -31,6 +31,19 @@ def test_library_exports_every_declared_symbol():
     assert lib.mtc_missing_symbols == []
 
 
+def test_header_constants_match_the_bindings():
+    """Every integer #define of the header that the Python bindings restate
+    carries the same value there."""
+    with open(HEADER) as f:
+        text = f.read()
+    defines = dict(re.findall(r"^#define\s+(MTC_[A-Z0-9_]+)\s+(-?\d+)\s*$", text, flags=re.M))
+    assert "MTC_GOD_RAYS_NOALLOC_SCRATCH_BYTES" in defines
+    shared = [k for k in defines if hasattr(A, k)]
+    assert len(shared) >= 4
+    for k in shared:
+        assert getattr(A, k) == int(defines[k]), k
+
+
 def test_std140_layouts():
     assert C.sizeof(A.MtcCamera) == 152
     assert A.MtcCamera.proj.offset == 64 and A.MtcCamera.eye.offset == 128 and A.MtcCamera.tan_fov_by2.offset == 144
@@ -133,8 +146,9 @@ def test_learned_light_structs_and_null_handling():
 
 def test_god_rays_noalloc_rejects_bad_input_on_the_host():
     """mtc_god_rays_noalloc validates everything before touching a device:
-    null pointers, non-positive sizes, sizes whose footprint map exceeds
-    the static scratch, and non-finite uniforms.  The pointers passed with
+    null pointers, non-positive sizes, sizes beyond the limits that keep
+    its decoded mask and blur table inside the static scratch, and
+    non-finite uniforms.  The pointers passed with
     bad sizes are never dereferenced."""
     lib = M.lib()
     cam, sun = A.MtcCamera(), A.MtcSun()
@@ -149,6 +163,15 @@ def test_god_rays_noalloc_rejects_bad_input_on_the_host():
     limit = A.MTC_GOD_RAYS_NOALLOC_MAX_ENTRIES
     assert f(2047, limit // 2048, C.byref(cam), C.byref(sun), dummy, dummy, None) == A.MTC_ERR_INVALID_ARG
     assert b"MTC_GOD_RAYS_NOALLOC_MAX_ENTRIES" in lib.mtc_last_error()
+    # few entries but more rows than the blur's row table holds
+    tall = A.MTC_GOD_RAYS_NOALLOC_MAX_HEIGHT + 1
+    assert 2 * (tall + 1) <= limit
+    assert f(1, tall, C.byref(cam), C.byref(sun), dummy, dummy, None) == A.MTC_ERR_INVALID_ARG
+    assert b"MTC_GOD_RAYS_NOALLOC_MAX_HEIGHT" in lib.mtc_last_error()
+    wide = A.MTC_GOD_RAYS_NOALLOC_MAX_WIDTH + 1
+    assert (wide + 1) * 2 <= limit
+    assert f(wide, 1, C.byref(cam), C.byref(sun), dummy, dummy, None) == A.MTC_ERR_INVALID_ARG
+    assert b"MTC_GOD_RAYS_NOALLOC_MAX_WIDTH" in lib.mtc_last_error()
     assert f(3840, 2160, C.byref(cam), C.byref(sun), dummy, dummy, None) == A.MTC_ERR_INVALID_ARG
     cam.eye[1] = float("nan")
     assert f(64, 32, C.byref(cam), C.byref(sun), dummy, dummy, None) == A.MTC_ERR_INVALID_ARG
