@@ -476,8 +476,10 @@ typedef struct mtc_mlp_train_desc {
 
 /* Held-out error of the network by predicted-value bin, and the fallback
  * rule a learned frame applies: a step goes to the cone when its local
- * density exceeds fallback_density or its prediction falls in a bin whose
- * RMS error exceeds max_rms. */
+ * density exceeds fallback_density (it then does not query the network)
+ * or its prediction falls in a bin whose RMS error exceeds max_rms.  The
+ * 16 bins cover [lo, hi] square-root spaced from the top: bin b holds
+ * predictions y with 16 sqrt((hi - y) / (hi - lo)) in [b, b + 1). */
 typedef struct mtc_mlp_calibration {
   float lo, hi;             /* bin range of the predicted value          */
   float rms[16];            /* held-out RMS error per bin                */

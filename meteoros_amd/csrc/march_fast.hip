@@ -259,7 +259,7 @@ struct FastViews {
   // learned light (MTC_KERNEL_LEARNED): network and its error calibration
   const bf16x8* mlp_frag;
   const float* mlp_params;
-  float calib_lo, calib_inv;  // bin = (y - lo) * inv, 16 bins
+  float calib_hi, calib_inv;  // calib_bin(y, hi, inv): 16 bins
   unsigned int calib_mask;    // bins whose held-out error is too high
   float fallback_density;     // local density above which the cone runs
   int audit;                  // also cone-march one in `audit` answered steps (power of two, 0 = off)
@@ -748,14 +748,16 @@ __device__ __forceinline__ void march_tile(const FrameConsts& fc, const FastView
     bool audit = false;
     float c0 = 0.0f;
     if (LEARN && nls > 0) {
-      // every in-cloud step queries the network (one row per step, ranked
-      // by ballot); steps it should not be trusted with go to the cone
-      const unsigned long long qbal = __ballot(pr.cloud);
+      // in-cloud steps at or below the fallback density query the network
+      // (one row per step, ranked by ballot); the others, and those whose
+      // prediction falls in an untrusted bin, go to the cone
+      const bool query = pr.cloud && !(pr.base / fc.base_density_factor > fv.fallback_density);
+      const unsigned long long qbal = __ballot(query);
       if (qbal) {
         uint16_t* X = mlp_lds + wave * 64 * 16;
         const int nq = __popcll(qbal);
         const int qrank = __popcll(qbal & ((1ull << lane) - 1ull));
-        if (pr.cloud) {
+        if (query) {
           c0 = cone_term<COV>(fc, fv, skern, pr, rm.cone_scale, 0);
           float f[9];
           mlp_features(fc, pr, rm, c0, f);
@@ -774,11 +776,9 @@ __device__ __forceinline__ void march_tile(const FrameConsts& fc, const FastView
         __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
         const float y = mlp::infer_wave(X, fv.mlp_frag, fv.mlp_params, nq);
         const float ym = __shfl(y, qrank, 64);
-        if (pr.cloud) {
-          int bin = (int)((ym - fv.calib_lo) * fv.calib_inv);
-          bin = bin < 0 ? 0 : (bin > 15 ? 15 : bin);
-          const bool fb = pr.base / fc.base_density_factor > fv.fallback_density || ((fv.calib_mask >> bin) & 1u);
-          if (!fb) {
+        if (query) {
+          const int bin = calib_bin(ym, fv.calib_hi, fv.calib_inv);
+          if (!((fv.calib_mask >> bin) & 1u)) {
             dal_mlp = c0 - ym;
             need_cone = false;
             audit = fv.audit && (step_hash((uint32_t)px, (uint32_t)py, (uint32_t)(k0 + j), 0u) & (uint32_t)(fv.audit - 1)) == 0u;
@@ -1368,7 +1368,7 @@ static mtc::fast::FastViews learned_views(const uint4* lf, const uint4* hf, cons
   mtc::fast::FastViews fv{lf, hf, curl, weather, vv.lf_shift, vv.hf_shift, vv.curl_shift, vv.weather_shift};
   fv.mlp_frag = (const bf16x8*)ll.frag;
   fv.mlp_params = ll.params;
-  fv.calib_lo = ll.calib_lo;
+  fv.calib_hi = ll.calib_hi;
   fv.calib_inv = ll.calib_inv;
   fv.calib_mask = ll.calib_mask;
   fv.fallback_density = ll.fallback_density;

@@ -130,8 +130,8 @@ int learned_light_of(const mtc_ctx* c, LearnedLight* out) {
   out->frag = m->frag;
   out->params = m->p;
   const mtc_mlp_calibration& k = m->calib;
-  out->calib_lo = k.lo;
-  out->calib_inv = k.hi > k.lo ? 16.0f / (k.hi - k.lo) : 0.0f;
+  out->calib_hi = k.hi;
+  out->calib_inv = k.hi > k.lo ? 1.0f / (k.hi - k.lo) : 0.0f;
   out->calib_mask = k.mask;
   out->fallback_density = k.fallback_density;
   out->audit = c->mlp_audit;
@@ -412,15 +412,14 @@ int mtc_mlp_calibrate(mtc_mlp* m, int64_t holdout, float max_rms, float fallback
   c.lo = *std::min_element(pred.begin(), pred.end());
   c.hi = *std::max_element(pred.begin(), pred.end());
   if (!(c.hi > c.lo)) c.hi = c.lo + 1.0f;
-  const double inv = 16.0 / ((double)c.hi - (double)c.lo);
+  const float inv = 1.0f / (c.hi - c.lo);
   double sum[16] = {0}, all = 0.0, mean = 0.0;
   for (int64_t i = 0; i < holdout; ++i) mean += y[i];
   mean /= (double)holdout;
   double var = 0.0;
   for (int64_t i = 0; i < holdout; ++i) {
     const double e = (double)pred[i] - (double)y[i];
-    int b = (int)(((double)pred[i] - c.lo) * inv);
-    b = b < 0 ? 0 : (b > 15 ? 15 : b);
+    const int b = calib_bin(pred[i], c.hi, inv);
     sum[b] += e * e;
     c.count[b]++;
     all += e * e;

@@ -203,11 +203,23 @@ struct TxaaConsts {
   int32_t width, height;
 };
 
+// Calibration bin of a network prediction y: 16 bins over [lo, hi],
+// square-root spaced from the top (bin b holds 16 sqrt((hi - y) / (hi - lo))
+// in [b, b + 1)), so the dense, small-magnitude predictions of thin cloud
+// get narrow bins of their own.  inv_span = 1 / (hi - lo).  The same FP32
+// operations on the host (calibration) and in the frame kernel.
+__host__ __device__ inline int calib_bin(float y, float hi, float inv_span) {
+  float t = (hi - y) * inv_span;
+  t = t > 0.0f ? t : 0.0f;
+  const int b = (int)(16.0f * sqrtf(t));
+  return b < 15 ? b : 15;
+}
+
 // Learned-light network as a frame kernel sees it.
 struct LearnedLight {
   const void* frag;       // packed BF16 MFMA weight fragments
   const float* params;    // FP32 parameter vector (biases, output layer)
-  float calib_lo, calib_inv;
+  float calib_hi, calib_inv;
   unsigned int calib_mask;
   float fallback_density;
   int audit;

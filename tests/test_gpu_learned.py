@@ -5,9 +5,11 @@
   * the training loop on a known smooth function;
   * the sample generator (the march writing (inputs, -(cone terms 1..5))
     per in-cloud step) against the CPU oracle's records of the same steps;
-  * what the trained network gains over the cone march in a 1080p frame:
-    cone work removed, march time, and PSNR vs the FP64 oracle next to the
-    fast kernel's and the zeroed network's (the cone's first term alone).
+  * what the trained network gains over the methods without it (held-out
+    error against dropping, averaging or extrapolating the cone terms it
+    replaces) and, in an unseen 1080p frame at the 45 dB bar against the
+    FP64 oracle, the share of steps it answers and the cone work it
+    removes, with both marches' times printed.
 
 Tolerances: inference within 1e-2 of the output scale (FP32 sums, BF16
 activations whose rounding may flip by one unit); gradient blocks within
@@ -198,26 +200,27 @@ def _frame(kernel, dims, cam, tm, sun, params, net=None, flags=0, frames=1):
 
 
 def test_learned_light_gain_over_cone_march():
-    """What the trained network gains over the methods without it that cost
-    the same cone work (one cone sample per in-cloud step, the network's
-    own input term 0), and what it costs against the full cone:
+    """What the trained network gains over the methods without it, and what
+    it costs against the full cone march:
 
     * on the held-out cone-march samples, the network's error in the five
       remaining cone terms against (a) dropping them (the network zeroed),
       (b) the mean predictor and (c) extrapolating the first term (rest =
       5 * term 0: the density at the step taken as constant along the
       cone);
-    * on an unseen 1080p frame, PSNR vs the FP64 oracle with the network
-      and with it zeroed, cone work and march time against the full cone
-      march of the fast kernel (which stays faster on MI355X: the six cone
-      samples it replaces are cheap L2-resident gathers, one query costs 192
-      SiLUs)."""
+    * on an unseen 1080p frame, at the calibration that keeps the frame at
+      the 45 dB bar against the FP64 oracle (the bar the cone march also
+      meets; prediction bins with held-out RMS above 0.09 go to the cone):
+      the share of in-cloud steps the network answers and the cone samples
+      it removes, with the march time of both printed.  On MI355X the
+      learned march is the slower one: a query costs 192 SiLUs (an exp and
+      a reciprocal each, at a quarter of the VALU rate) against five cone
+      samples that are L2-resident gathers."""
     dims = (128, 32, 128, 512)
     with M.LightNet(capacity=1 << 22) as net:
         n, curve, cal = _train_on_scenes(net, dims)
         print(f"trained on {n} samples: loss {curve[:20].mean():.4f} -> {curve[-100:].mean():.5f}; held-out rms "
-              f"{cal['rms_all']:.4f} (mean predictor {cal['rms_mean_predictor']:.4f}); bins sent to the cone "
-              f"{bin(cal['mask'])}")
+              f"{cal['rms_all']:.4f} (mean predictor {cal['rms_mean_predictor']:.4f})")
         hold = min(65536, n // 8)
         hx, hy, _ = net.read_samples(n - hold, hold)
         rms_zero = float(np.sqrt(np.mean(hy.astype(np.float64) ** 2)))
@@ -226,16 +229,15 @@ def test_learned_light_gain_over_cone_march():
               f"mean predictor {cal['rms_mean_predictor']:.4f}, first term extrapolated {rms_extrap:.4f}")
         assert cal["rms_all"] < 0.6 * cal["rms_mean_predictor"]
         assert cal["rms_all"] < rms_extrap and cal["rms_all"] < rms_zero
+        bar = net.calibrate(holdout=hold, max_rms=0.09, fallback_density=0.8)
+        print(f"calibration at max_rms 0.09: bins to the cone {bin(bar['mask'])}, held-out rms per bin "
+              f"{np.round(bar['rms'], 3).tolist()}")
         # an unseen pose of the 1080p workload
         cam, tm, sun, params = _scene(1920, 1080, 22.0, 15.0, frame=5)
         fast, vols = _frame(A.MTC_KERNEL_FAST, dims, cam, tm, sun, params, flags=A.MTC_FLAG_COUNTERS)
         learned, _ = _frame(A.MTC_KERNEL_LEARNED, dims, cam, tm, sun, params, net, flags=A.MTC_FLAG_COUNTERS)
         tf, _ = _frame(A.MTC_KERNEL_FAST, dims, cam, tm, sun, params, flags=A.MTC_FLAG_TIMING, frames=20)
         tl, _ = _frame(A.MTC_KERNEL_LEARNED, dims, cam, tm, sun, params, net, flags=A.MTC_FLAG_TIMING, frames=20)
-        trained_p = net.params
-        net.params = np.zeros_like(trained_p)  # network output 0: the cone's first term alone
-        first_only, _ = _frame(A.MTC_KERNEL_LEARNED, dims, cam, tm, sun, params, net)
-        net.params = trained_p
     vol = O.Volumes(*vols, *dims, 0)
     W, H = 1920, 1080
     xy = O.select_pixels(W, H, 5)[::4]
@@ -244,22 +246,21 @@ def test_learned_light_gain_over_cone_march():
     pick = lambda r: r[xy[:, 1], xy[:, 0], :3]  # noqa: E731
     p_fast = O.psnr_vs_fp64(pick(fast["rgba"]), ref[:, :3], keep)
     p_learned = O.psnr_vs_fp64(pick(learned["rgba"]), ref[:, :3], keep)
-    p_first = O.psnr_vs_fp64(pick(first_only["rgba"]), ref[:, :3], keep)
     cf, cl = fast["counters"], learned["counters"]
     answered = cl["mlp_queries"] - cl["mlp_fallbacks"]
     audit_rms = np.sqrt(cl["mlp_audit_sq_err"] * 1e-6 / max(1, cl["mlp_audits"]))
-    print(f"fast:    {tf['march_ms']:.4f} ms march, cone samples {cf['cone_samples']}, PSNR {p_fast:.2f} dB")
-    print(f"learned: {tl['march_ms']:.4f} ms march, cone samples {cl['cone_samples']}, queries {cl['mlp_queries']} "
-          f"(answered {answered}, to the cone {cl['mlp_fallbacks']}), audit rms {audit_rms:.4f} over "
-          f"{cl['mlp_audits']} audits, PSNR {p_learned:.2f} dB")
-    print(f"first cone term only (network zeroed): PSNR {p_first:.2f} dB")
+    print(f"cone march (fast kernel): {tf['march_ms']:.4f} ms march, cone samples {cf['cone_samples']}, "
+          f"PSNR {p_fast:.2f} dB")
+    print(f"learned:  {tl['march_ms']:.4f} ms march, cone samples {cl['cone_samples']}, in-cloud steps "
+          f"{cl['mlp_queries']} (answered by the network {answered}, to the cone {cl['mlp_fallbacks']}), audit rms "
+          f"{audit_rms:.4f} over {cl['mlp_audits']} audits, PSNR {p_learned:.2f} dB")
     # the same rays, the same steps: both kernels see the same in-cloud steps
     assert cl["cloud_steps"] == cf["cloud_steps"]
     assert cl["mlp_queries"] == cl["cloud_steps"]
-    # the network answers most steps, so most of the cone work is gone ...
-    assert answered >= 0.5 * cl["mlp_queries"]
-    assert cl["cone_samples"] <= 0.6 * cf["cone_samples"]
-    # ... the cone's audits see the error the held-out set predicted ...
-    assert cl["mlp_audits"] > 0 and audit_rms < 2.0 * cal["rms_all"] + 0.02
-    # ... and the trained network is what makes the answers usable
-    assert p_learned > p_first + 3.0
+    # at the bar both methods meet ...
+    assert p_fast >= 45.0 and p_learned >= 45.0
+    # ... the network answers a share of the steps and removes that share of the cone's work ...
+    assert answered >= 0.1 * cl["mlp_queries"]
+    assert cl["cone_samples"] <= cf["cone_samples"] - 4 * answered
+    # ... and the cone's audits see the error the calibration admitted
+    assert cl["mlp_audits"] > 0 and audit_rms < 2.0 * 0.09 + 0.02
