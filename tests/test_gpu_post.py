@@ -77,6 +77,26 @@ def test_tone_map_matches_oracle(as_float):
         assert np.all(got[..., 3] == 255)
 
 
+@pytest.mark.parametrize("W,H,pitch", [(65, 37, 85.0), (1, 1, 75.0), (130, 17, 60.0), (200, 300, 89.0)])
+def test_god_rays_odd_sizes_match_oracle(W, H, pitch):
+    """The blur's workgroups cover 64 columns x 16 rows: partial waves (65,
+    130, 200 columns), a 1x1 image, partial row groups (37, 17, 300 rows) and
+    the sun inside the image (pitch 75-89 deg puts it at 19-48% of the
+    height) must all match the oracle."""
+    st, cam, tm, sun = _frame(W, H, pitch)
+    with st:
+        rgba, mask = st.read_output()
+        st.god_rays(cam, sun)
+        got, _ = st.read_output()
+    ref, (sx, sy, blend) = O.god_rays(cam, sun, rgba, mask, precision=32)
+    peak = float(np.abs(ref).max())
+    err = np.abs(got.astype(np.float64) - ref)
+    print(f"{W}x{H} pitch {pitch}: sun_ss ({sx:.3f}, {sy:.3f}) blend {blend:.3f}; max err {err.max():.2e} "
+          f"(peak {peak:.2f})")
+    assert blend > 0.0
+    assert err.max() <= 1e-5 * peak
+
+
 def test_god_rays_then_tone_map_at_1080p():
     W, H = 1920, 1080
     st, cam, tm, sun = _frame(W, H, 55.0)
