@@ -66,6 +66,8 @@ __device__ __forceinline__ float decode(float4 c) {
 __global__ __launch_bounds__(256) void god_rays_setup_kernel(PostConsts pc, const float4* __restrict__ mask,
                                                              int table_blocks, float* __restrict__ map,
                                                              GodRayEntry* __restrict__ tab) {
+  static_assert(kGodRayGroupRows * 4 == 64 && kGodRaySamples == 4 * 25,
+                "one wave per row group: 16 rows x 4 chunks of 25 samples");
   const int W = pc.width, H = pc.height;
   if ((int)blockIdx.x < table_blocks) {
     const int g = (int)blockIdx.x * 4 + (int)(threadIdx.x >> 6);
