@@ -260,11 +260,17 @@ def run_batch(kernel: int, renders: int = 5, check_rays: int = 0) -> dict:
     return out
 
 
-# Algorithmic HBM bytes per pixel of the post passes: god rays = mask read +
-# footprint write (decode) + footprint, image read + image write (100 taps
-# along the sun line hit the footprint in cache); tone map = RGBA32F read +
-# RGBA8 write; TXAA = RGBA8 frame read + RGBA8 history read + RGBA8 write.
-POST_BYTES_PER_PX = {"god_rays": 80, "tone_map": 20, "txaa": 12}
+# Algorithmic HBM bytes per pixel of the post passes: god rays = RGBA32F mask
+# read + decoded float map write and read once (the 100 taps along the sun
+# line are served by the workgroups' LDS staging and the caches) + image
+# read + image write (the blur's table adds under 1 B/px); tone map =
+# RGBA32F read + RGBA8 write; TXAA = RGBA8 frame read + RGBA8 history read +
+# RGBA8 write.  `bound` names what limits each pass as measured
+# (profiles/god_rays_1080p_counters.md, profiles/march_kernels_1080p_kernel_stats.md):
+# only the tone map is priced by its bytes.
+POST_BYTES_PER_PX = {"god_rays": 56, "tone_map": 20, "txaa": 12}
+POST_BOUND = {"god_rays": "texture data path (LDS staging loads) and VALU, not HBM", "tone_map": "hbm",
+              "txaa": "valu (about 20 correctly rounded divisions per pixel)"}
 
 
 def run_post(kernel, seq, warmup: int, reps: int = 100) -> dict:
@@ -289,7 +295,8 @@ def run_post(kernel, seq, warmup: int, reps: int = 100) -> dict:
             us = 1e6 * (time.perf_counter() - t0) / reps
             gbs = POST_BYTES_PER_PX[name] * W * H / (us * 1e-6) / 1e9
             out[name] = {"us_per_pass": round(us, 2), "bytes_per_px": POST_BYTES_PER_PX[name],
-                         "gb_per_s": round(gbs, 1), "frac_of_8tbs": round(gbs / 8000.0, 4)}
+                         "gb_per_s": round(gbs, 1), "frac_of_8tbs": round(gbs / 8000.0, 4),
+                         "bound": POST_BOUND[name]}
     finally:
         st.close()
     return out
