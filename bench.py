@@ -512,8 +512,9 @@ def main() -> int:
 
     if args.no_cpu_baseline:
         # no checker in this run: pick among the kernels the GPU tests hold to
-        # the PSNR bar (tests/test_gpu_fast.py)
-        valid = [n for n in names if n in ("fast", "precision", "fast_bricks", "precision_bricks")]
+        # the PSNR bar (tests/test_gpu_fast.py, tests/test_gpu_bricks.py,
+        # tests/test_gpu_learned.py)
+        valid = [n for n in names if n in ("fast", "precision", "fast_bricks", "precision_bricks", "learned")]
     else:
         valid = [n for n in names if results[n]["acc"]["psnr_db_fp64"] >= PSNR_BAR_DB]
     if not valid:
