@@ -215,12 +215,32 @@ def batch_views(seed: int = 11):
     return cams, times, suns, params
 
 
+def batch_work(cams, times, suns, params) -> dict:
+    """Counted work of the 96-view batch: each view marched on its own (every
+    pixel, as the batch marches them) by the plain kernel with counters, one
+    context per view size, the same seeded volumes."""
+    total: dict = {}
+    for size in sorted(set(BATCH_SIZES)):
+        with M.CloudStage(*size, M.VolumeSpec(128, 32, 128, 512, A.MTC_FMT_UNORM8), kernel=A.MTC_KERNEL_PLAIN,
+                          flags=A.MTC_FLAG_COUNTERS) as st:
+            st.generate_volumes(1, 2)
+            st.reset_counters()
+            for v, wh in enumerate(BATCH_SIZES):
+                if wh == size:
+                    st.march_full(cams[v], times[v], suns[v], params[v])
+            st.synchronize()
+            for k, x in st.counters().items():
+                total[k] = total.get(k, 0) + x
+    return total
+
+
 def run_batch(kernel: int, renders: int = 5, check_rays: int = 0) -> dict:
     """Autotunes and times the 96-view batch (one persistent launch in a
     hipGraph per render, uniforms uploaded each render).  With check_rays >
     0 the CPU leg then checks every view of the last render against the FP64
     oracle on a strided subset of about check_rays pixels and reports the
-    minimum and median PSNR per size class."""
+    minimum and median PSNR per size class.  The roofline divides the
+    counted FP32 work of the 96 views by the launch's hipEvent duration."""
     cams, times, suns, params = batch_views()
     views = None
     with M.CloudStage(16, 16, M.VolumeSpec(128, 32, 128, 512, A.MTC_FMT_UNORM8), kernel=kernel) as st:
@@ -234,12 +254,26 @@ def run_batch(kernel: int, renders: int = 5, check_rays: int = 0) -> dict:
                 b.render(cams, times, suns, params)
             st.synchronize()
             dt = (time.perf_counter() - t0) / renders
+            launch_ms = []
+            for _ in range(renders):
+                b.render(cams, times, suns, params)
+                launch_ms.append(b.last_ms())
             if check_rays > 0:
                 views = [b.read(v)[0] for v in range(len(BATCH_SIZES))]
                 vols = st.get_volumes()
     px = sum(w * h for w, h in BATCH_SIZES)
     out = {"views": len(BATCH_SIZES), "pixels": px, "tuning": best, "autotune_best_ms": round(tune_ms, 3),
            "ms_per_batch": round(1000.0 * dt, 3), "mpixels_per_s": round(px / dt / 1e6, 1), "renders": renders}
+    work = batch_work(cams, times, suns, params)
+    kms = float(np.median(launch_ms))
+    flops = march_flops(work)
+    tf = flops / (kms * 1e-3) / 1e12
+    out["ray_steps"] = work["primary_steps"]
+    out["ray_steps_per_s"] = work["primary_steps"] / dt
+    out["roofline"] = {"bound": "valu", "pipe": "valu-fp32", "kernel": "batch march (one persistent launch)",
+                       "achieved": round(tf, 3), "peak": 157.3, "unit": "TFLOP/s", "frac": round(tf / 157.3, 4),
+                       "flops_per_launch": flops, "avg_launch_ms": round(kms, 4),
+                       "launch_ms_from": "hipEvents around the render's graph launch, median of the renders"}
     if views is not None:
         from oracle import oracle as O
 
