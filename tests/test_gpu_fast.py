@@ -172,6 +172,39 @@ def test_results_do_not_depend_on_lanes_per_ray(kernel):
         assert cnt == outs[0][2]
 
 
+@pytest.mark.parametrize("W,H,full,lanes", [(416, 172, False, 8), (416, 172, False, 4), (300, 77, True, 8),
+                                             (1920, 1080, False, 8), (36, 20, True, 2)])
+def test_strip_order_marches_every_ray_once(W, H, full, lanes):
+    """The fast launch deals its tiles to the XCDs in column strips (a
+    different tile order per image width, with rows past the last whole
+    block of strips in raster order).  Every ray of the frame must be marched
+    exactly once: the counters' marched + ocean + sky rays equal the frame's
+    rays (a tile dealt twice or never would move the sum by a tile's rays),
+    and every one of them holds a written colour (the march writes alpha 1
+    on every ray it visits; the context's images start as zeros)."""
+    params, sun = M.preset_params(2)
+    cam = ground_camera(W, H, 10.0, 25.0).ubo()
+    tm = _time(5)
+    st = M.CloudStage(W, H, M.VolumeSpec(128, 32, 128, 512), kernel=A.MTC_KERNEL_PRECISION,
+                      flags=A.MTC_FLAG_COUNTERS, lanes_per_ray=lanes)
+    st.generate_volumes(1, 2)
+    if full:
+        st.march_full(cam, tm, sun, params)
+        n_rays = W * H
+    else:
+        st.frame(cam, cam, tm, sun, params)
+        n_rays = len(O.select_pixels(W, H, 5))
+    cnt = st.counters()
+    rgba = st.read_output()[0]
+    st.close()
+    assert cnt["rays_marched"] + cnt["rays_ocean"] + cnt["rays_sky"] == n_rays
+    if full:
+        assert np.all(rgba[..., 3] == 1.0)
+    else:
+        xy = O.select_pixels(W, H, 5)
+        assert np.all(rgba[xy[:, 1], xy[:, 0], 3] == 1.0)
+
+
 @pytest.mark.parametrize("kernel", [A.MTC_KERNEL_FAST, A.MTC_KERNEL_PRECISION, A.MTC_KERNEL_LEARNED])
 def test_march_history_changes_no_pixel(kernel):
     """Whatever a context keeps from march to march (the learned march deals
