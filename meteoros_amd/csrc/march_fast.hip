@@ -1056,7 +1056,10 @@ __device__ __forceinline__ void strip_tile(int b, int tiles_x, int tiles_y, int 
 
 // One frame: a one-wave workgroup marches an 8 x (8 / G) tile of rays (a
 // finer grain than four-wave groups: the launch drains in smaller steps),
-// in strip order (above).  Dealing tiles longest first (as the learned
+// in strip order (above).  PAIR: cone rounds two at a time (95 VGPRs, 5
+// waves per SIMD), for launches of a few tiles per wave slot, whose time
+// is set by how soon the last tiles finish; single rounds (78 VGPRs, 6
+// waves per SIMD) for longer launches, whose time is set by throughput.  Dealing tiles longest first (as the learned
 // kernel does) scatters the resident waves over the image, and the lost
 // texel locality costs more than the shorter tail gains.
 //
@@ -1066,8 +1069,8 @@ __device__ __forceinline__ void strip_tile(int b, int tiles_x, int tiles_y, int 
 // they run in the wave slots the draining march leaves free, and the frame
 // needs no second stream and no fork / join events.  The two parts write
 // disjoint pixels and the march reads nothing the reprojection writes.
-template <int COV, bool F16, bool FP64SHELL, bool COUNT, int G, bool BRICK>
-__global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(COUNT ? 1 : 5))) void march_kernel(
+template <int COV, bool F16, bool FP64SHELL, bool COUNT, int G, bool BRICK, bool PAIR = true>
+__global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(COUNT ? 1 : (PAIR ? 5 : 6)))) void march_kernel(
     FrameConsts fc, FastViews fv, OutputViews ov, int full, int tiles_x, int n_tiles, int nstrip, int strip_k) {
   const int b = (int)blockIdx.x;
   if (b >= n_tiles) {
@@ -1088,7 +1091,7 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(COUNT ? 1 : 
   constexpr int TX = 8, TY = 64 / G / 8;
   int tx, ty;
   strip_tile(b, tiles_x, n_tiles / tiles_x, nstrip, strip_k, tx, ty);
-  march_tile<COV, F16, FP64SHELL, COUNT, G, false, true, BRICK>(fc, fv, ov.color, ov.mask, ov.counters, full,
+  march_tile<COV, F16, FP64SHELL, COUNT, G, false, PAIR, BRICK>(fc, fv, ov.color, ov.mask, ov.counters, full,
                                                                 tx * TX, ty * TY, TX, sjit, skern, cone_lds, nullptr,
                                                                 lf_box);
 }
@@ -1177,8 +1180,15 @@ __global__ __launch_bounds__(256) void light_samples_kernel(FrameConsts fc, Fast
 }
 
 // Many views in one launch: a persistent grid pulls (view, tile) work items
-// from a queue (items sorted by estimated cost when the batch was built).
-// Each view carries its own FrameConsts, size, output offset and steps.
+// from a queue (items sorted by estimated cost when the batch was built),
+// kBatchClaim consecutive items per atomic add: every add on the one
+// counter is a device-scope atomic that the memory side serialises, and
+// its round trip is exposed before the tile starts.  Measured on MI355X
+// (96-view set, precision): 1 item per add 18.36 ms, 2 18.04, 4 17.97, 8
+// 17.95.  Each view carries its own FrameConsts, size, output offset and
+// steps.
+constexpr int kBatchClaim = 4;
+
 template <int COV, bool F16, int G>
 __global__ __launch_bounds__(256) void march_batch_kernel(const BatchView* __restrict__ views,
                                                           const BatchItem* __restrict__ items, int n_items,
@@ -1190,10 +1200,14 @@ __global__ __launch_bounds__(256) void march_batch_kernel(const BatchView* __res
   __shared__ ConeLds cone_lds[4];
   __shared__ int s_item;
   const int tile_y = 256 / G / tile_x;
+  int it = 0, left = 0;  // next item of the current claim, items left in it
   for (;;) {
-    if (threadIdx.x == 0) s_item = (int)atomicAdd(queue, 1u);
-    __syncthreads();
-    const int it = s_item;
+    if (left == 0) {
+      if (threadIdx.x == 0) s_item = (int)atomicAdd(queue, (unsigned)kBatchClaim);
+      __syncthreads();
+      it = s_item;
+      left = kBatchClaim;
+    }
     if (it >= n_items) break;
     const BatchItem wi = items[it];
     const BatchView& bv = views[__builtin_amdgcn_readfirstlane(wi.view)];
@@ -1203,6 +1217,8 @@ __global__ __launch_bounds__(256) void march_batch_kernel(const BatchView* __res
     const int tx = wi.tile % bv.tiles_x, ty = wi.tile / bv.tiles_x;
     march_tile<COV, F16, true, false, G>(bv.fc, fv, color + bv.offset, mask + bv.offset, nullptr, 1, tx * tile_x,
                                          ty * tile_y, tile_x, sjit, skern, cone_lds);
+    ++it;
+    --left;
     // the next item rewrites the tables and the queue slot
     __syncthreads();
   }
@@ -1334,6 +1350,27 @@ hipError_t pack_fast_resources(const VolumeViews& vv, uint4* lf, uint4* hf, uint
 struct StripOrder {
   int nstrip, k;
 };
+
+// Launches with at least this many march tiles per compute unit use the
+// single-round kernel.  Measured on MI355X (precision march, tiles per CU:
+// paired-round time -> single-round time): 1080p 4x4 frames (63) 0.2029 ->
+// 0.2066 ms; 2560x1440 4x4 (112) 0.346 -> 0.342 ms; 3200x1800 4x4 (176)
+// 0.516 -> 0.504 ms; 4K 4x4 (253) 0.875 -> 0.847 ms; 1080p full (1013) 2.16
+// -> 2.12 ms.
+constexpr int kSingleRoundTilesPerCu = 96;
+
+// Compute units of the current device (cached per device).
+static int compute_units() {
+  static int cached[64] = {};
+  int dev = 0;
+  if (hipGetDevice(&dev) != hipSuccess || dev < 0 || dev >= 64) return 256;
+  if (!cached[dev]) {
+    int cus = 0;
+    if (hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess || cus <= 0) cus = 256;
+    cached[dev] = cus;
+  }
+  return cached[dev];
+}
 static StripOrder strip_order(int tiles_x) {
   StripOrder o;
   o.nstrip = 8 * std::max(1, (tiles_x + 10) / 20);
@@ -1357,9 +1394,13 @@ static void launch_v(hipStream_t s, const FrameConsts& fc, const mtc::fast::Fast
   const int strips = with_reproj ? ((fc.width + 63) / 64) * ((fc.height + 3) / 4) : 0;
   const StripOrder so = strip_order(tiles_x);
   dim3 grid((unsigned)(n_tiles + strips)), block(64);
+  constexpr bool kSingle = !BRICK && (G == 4 || G == 8);  // single-round kernels built for these
   if (ov.counters)
     launch_timed(mtc::fast::march_kernel<COV, F16, FP64, true, G, BRICK>, grid, block, s, ke, fc, fv, ov, full,
                  tiles_x, n_tiles, so.nstrip, so.k);
+  else if (kSingle && n_tiles >= kSingleRoundTilesPerCu * compute_units())
+    launch_timed(mtc::fast::march_kernel<COV, F16, FP64, false, G, BRICK, !kSingle>, grid, block, s, ke, fc, fv, ov,
+                 full, tiles_x, n_tiles, so.nstrip, so.k);
   else
     launch_timed(mtc::fast::march_kernel<COV, F16, FP64, false, G, BRICK>, grid, block, s, ke, fc, fv, ov, full,
                  tiles_x, n_tiles, so.nstrip, so.k);

@@ -205,6 +205,32 @@ def test_strip_order_marches_every_ray_once(W, H, full, lanes):
         assert np.all(rgba[xy[:, 1], xy[:, 0], 3] == 1.0)
 
 
+@pytest.mark.parametrize("kernel", [A.MTC_KERNEL_FAST, A.MTC_KERNEL_PRECISION])
+def test_single_and_paired_cone_rounds_agree(kernel):
+    """Launches of few tiles per compute unit run the cone rounds two at a
+    time (5 waves per SIMD), longer ones one at a time (6 waves per SIMD).
+    A 1920x1080 4x4 frame (16 200 tiles) takes the paired kernel and the
+    full march of the same view (259 200 tiles) the single-round one; a
+    pixel's result depends only on its ray and the frame's uniforms, so the
+    frame's marched pixels must be the same bits in both."""
+    W, H = 1920, 1080
+    params, sun = M.preset_params(2)
+    cam = ground_camera(W, H, 15.0, 20.0).ubo()
+    tm = _time(3)
+    spec = M.VolumeSpec(128, 32, 128, 512)
+    with M.CloudStage(W, H, spec, kernel=kernel) as a, M.CloudStage(W, H, spec, kernel=kernel) as b:
+        a.generate_volumes(1, 2)
+        b.generate_volumes(1, 2)
+        a.frame(cam, cam, tm, sun, params)
+        b.march_full(cam, tm, sun, params)
+        fa, ma = a.read_output()
+        fb, mb = b.read_output()
+    xy = O.select_pixels(W, H, 3)
+    bits = lambda x: np.ascontiguousarray(x[xy[:, 1], xy[:, 0]]).view(np.uint32)  # noqa: E731
+    assert np.array_equal(bits(fa), bits(fb))
+    assert np.array_equal(bits(ma), bits(mb))
+
+
 @pytest.mark.parametrize("kernel", [A.MTC_KERNEL_FAST, A.MTC_KERNEL_PRECISION, A.MTC_KERNEL_LEARNED])
 def test_march_history_changes_no_pixel(kernel):
     """Whatever a context keeps from march to march (the learned march deals
