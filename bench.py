@@ -306,7 +306,7 @@ def run_batch(kernel: int, renders: int = 5, check_rays: int = 0) -> dict:
 # only the tone map is priced by its bytes.
 POST_BYTES_PER_PX = {"god_rays": 56, "tone_map": 20, "txaa": 12}
 POST_BOUND = {"god_rays": "texture data path (LDS staging loads) and VALU, not HBM", "tone_map": "hbm",
-              "txaa": "valu (about 20 correctly rounded divisions per pixel)"}
+              "txaa": "valu (ray, shell hit and re-projection per pixel; 9-tap neighbourhood box and clip)"}
 
 
 def run_post(kernel, seq, warmup: int, reps: int = 100) -> dict:

@@ -309,8 +309,28 @@ __device__ __forceinline__ float4 f4add(float4 a, float4 b) {
   return make_float4(a.x + b.x, a.y + b.y, a.z + b.z, a.w + b.w);
 }
 __device__ __forceinline__ float4 f4scale(float4 a, float s) { return make_float4(a.x * s, a.y * s, a.z * s, a.w * s); }
-__device__ __forceinline__ float4 f4div(float4 a, float s) {
-  return make_float4(a.x / s, a.y / s, a.z / s, a.w / s);
+
+// a / b through the hardware reciprocal (about 1 ulp) and one FMA
+// correction: q = a y, r = a - q b exactly, q + r y.  Four VALU operations
+// instead of the IEEE division's dozen; the result is the correctly rounded
+// quotient except in rare cases, where it is one ulp off.  For finite,
+// normal, non-zero b (every denominator of the TXAA pass: image sizes,
+// lengths, 2A of the shell quadratic, the projected depth, the clip box's
+// half extents (>= 1e-10) and the luminance floor (>= 0.2)).
+struct Recip {
+  float b, y;
+};
+__device__ __forceinline__ Recip recip(float b) { return Recip{b, __builtin_amdgcn_rcpf(b)}; }
+__device__ __forceinline__ float divr(float a, const Recip& d) {
+  const float q = a * d.y;
+  const float r = __builtin_fmaf(-q, d.b, a);
+  return __builtin_fmaf(r, d.y, q);
+}
+__device__ __forceinline__ float divr(float a, float b) { return divr(a, recip(b)); }
+__device__ __forceinline__ f3 divr(f3 a, const Recip& d) { return mk(divr(a.x, d), divr(a.y, d), divr(a.z, d)); }
+__device__ __forceinline__ f3 normalize_r(f3 a) { return divr(a, recip(length(a))); }
+__device__ __forceinline__ float4 f4divr(float4 a, const Recip& d) {
+  return make_float4(divr(a.x, d), divr(a.y, d), divr(a.z, d), divr(a.w, d));
 }
 
 // imageLoad of the current RGBA8 frame; outside the image it returns 0
@@ -343,7 +363,7 @@ __global__ __launch_bounds__(256) void txaa_kernel(TxaaConsts tc, const uint32_t
   const int px = blockIdx.x * 64 + (threadIdx.x & 63);
   const int py = blockIdx.y * 4 + (threadIdx.x >> 6);
   if (px >= W || py >= H) return;
-  const float u = ((float)px + 0.5f) / (float)W, v = ((float)py + 0.5f) / (float)H;
+  const float u = divr((float)px + 0.5f, (float)W), v = divr((float)py + 0.5f, (float)H);
   // castRay (:87-116) with the basis folded per frame
   float nx = u * 2.0f - 1.0f, ny = v * 2.0f - 1.0f;
   nx = nx + tc.jit[0];
@@ -351,7 +371,7 @@ __global__ __launch_bounds__(256) void txaa_kernel(TxaaConsts tc, const uint32_t
   const f3 eye = ld3(tc.eye);
   const f3 cx = (nx * tc.tfx) * ld3(tc.right), cy = (ny * tc.tfy) * ld3(tc.up);
   const f3 pnt = (ld3(tc.ref) + cx) + cy;
-  const f3 rD = normalize(pnt - eye);
+  const f3 rD = normalize_r(pnt - eye);
   // raySphereIntersection (:119-165) with the inner atmosphere of the TXAA shader
   const f3 rO = ld3(tc.ro);
   f3 P = mk(0.0f, 0.0f, 0.0f);
@@ -359,8 +379,9 @@ __global__ __launch_bounds__(256) void txaa_kernel(TxaaConsts tc, const uint32_t
     const float A = dot(rD, rD), B = 2.0f * dot(rD, rO), C = tc.C;
     const float disc = B * B - 4.0f * A * C;
     if (!(disc < 0.0f)) {
-      float t = (-B - __builtin_sqrtf(disc)) / (2.0f * A);
-      if (t < 0.0f) t = (-B + __builtin_sqrtf(disc)) / (2.0f * A);
+      const Recip a2 = recip(2.0f * A);
+      float t = divr(-B - __builtin_sqrtf(disc), a2);
+      if (t < 0.0f) t = divr(-B + __builtin_sqrtf(disc), a2);
       if (t >= 0.0f) P = ((rO + t * rD) * tc.R) + ld3(tc.center);
     }
   }
@@ -368,10 +389,10 @@ __global__ __launch_bounds__(256) void txaa_kernel(TxaaConsts tc, const uint32_t
   f3 q = mk(((m[0] * P.x + m[4] * P.y) + m[8] * P.z) + m[12] * 1.0f,
             ((m[1] * P.x + m[5] * P.y) + m[9] * P.z) + m[13] * 1.0f,
             ((m[2] * P.x + m[6] * P.y) + m[10] * P.z) + m[14] * 1.0f);
-  f3 od = normalize(q);
+  f3 od = normalize_r(q);
   const float nz = -od.z;
-  od = od / nz;
-  const float ou = (od.x / tc.tfx) * 0.5f + 0.5f, ov = (od.y / tc.tfy) * 0.5f + 0.5f;
+  od = divr(od, recip(nz));
+  const float ou = divr(od.x, tc.tfx) * 0.5f + 0.5f, ov = divr(od.y, tc.tfy) * 0.5f + 0.5f;
   // neighbourHoodClamping (:191-220): 3x3 and cross-5 min / max / mean
   const float4 ctl = load_cur(cur, W, H, px - 1, py - 1), ctc = load_cur(cur, W, H, px, py - 1),
                ctr = load_cur(cur, W, H, px + 1, py - 1), cml = load_cur(cur, W, H, px - 1, py),
@@ -380,12 +401,12 @@ __global__ __launch_bounds__(256) void txaa_kernel(TxaaConsts tc, const uint32_t
                cbr = load_cur(cur, W, H, px + 1, py + 1);
   float4 cmin = f4min(ctl, f4min(ctc, f4min(ctr, f4min(cml, f4min(cmc, f4min(cmr, f4min(cbl, f4min(cbc, cbr))))))));
   float4 cmax = f4max(ctl, f4max(ctc, f4max(ctr, f4max(cml, f4max(cmc, f4max(cmr, f4max(cbl, f4max(cbc, cbr))))))));
-  float4 cavg = f4div(f4add(f4add(f4add(f4add(f4add(f4add(f4add(f4add(ctl, ctc), ctr), cml), cmc), cmr), cbl), cbc),
-                            cbr),
-                      9.0f);
+  float4 cavg = f4divr(f4add(f4add(f4add(f4add(f4add(f4add(f4add(f4add(ctl, ctc), ctr), cml), cmc), cmr), cbl), cbc),
+                             cbr),
+                       recip(9.0f));
   const float4 cmin5 = f4min(ctc, f4min(cml, f4min(cmc, f4min(cmr, cbc))));
   const float4 cmax5 = f4max(ctc, f4max(cml, f4max(cmc, f4max(cmr, cbc))));
-  const float4 cavg5 = f4div(f4add(f4add(f4add(f4add(ctc, cml), cmc), cmr), cbc), 5.0f);
+  const float4 cavg5 = f4divr(f4add(f4add(f4add(f4add(ctc, cml), cmc), cmr), cbc), recip(5.0f));
   cmin = f4scale(f4add(cmin, cmin5), 0.5f);
   cmax = f4scale(f4add(cmax, cmax5), 0.5f);
   cavg = f4scale(f4add(cavg, cavg5), 0.5f);
@@ -398,13 +419,16 @@ __global__ __launch_bounds__(256) void txaa_kernel(TxaaConsts tc, const uint32_t
     const float ex = 0.5f * (cmax.x - cmin.x) + 1e-10f, ey = 0.5f * (cmax.y - cmin.y) + 1e-10f,
                 ez = 0.5f * (cmax.z - cmin.z) + 1e-10f;
     const float vx = prv.x - pcx, vy = prv.y - pcy, vz = prv.z - pcz, vw = prv.w - pw;
-    const float ma = fmaxf(fabsf(vx / ex), fmaxf(fabsf(vy / ey), fabsf(vz / ez)));
-    if (ma > 1.0f) prv = make_float4(pcx + vx / ma, pcy + vy / ma, pcz + vz / ma, pw + vw / ma);
+    const float ma = fmaxf(fabsf(divr(vx, ex)), fmaxf(fabsf(divr(vy, ey)), fabsf(divr(vz, ez))));
+    if (ma > 1.0f) {
+      const Recip rm = recip(ma);
+      prv = make_float4(pcx + divr(vx, rm), pcy + divr(vy, rm), pcz + divr(vz, rm), pw + divr(vw, rm));
+    }
   }
   // luminance-weighted feedback (:250-258)
   const float lum0 = (curr.x * 0.2125f + curr.y * 0.7154f) + curr.z * 0.0721f;
   const float lum1 = (prv.x * 0.2125f + prv.y * 0.7154f) + prv.z * 0.0721f;
-  const float diff = fabsf(lum0 - lum1) / fmaxf(lum0, fmaxf(lum1, 0.2f));
+  const float diff = divr(fabsf(lum0 - lum1), fmaxf(lum0, fmaxf(lum1, 0.2f)));
   const float w = 1.0f - diff;
   const float k = mixf(0.0f, 0.5f, w * w);
   const float4 o = make_float4(mixf(prv.x, curr.x, k), mixf(prv.y, curr.y, k), mixf(prv.z, curr.z, k),
