@@ -80,24 +80,43 @@ __device__ __forceinline__ void reproject_pixel(const FrameConsts& fc, const Out
   const float fW = (float)W, fH = (float)H, maxx = (float)(W - 1), maxy = (float)(H - 1);
   float4 acc = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
   const size_t o = (size_t)y * (size_t)W + (size_t)x;
-  for (int i = 0; i < N; ++i) {
-    const float fr = fc.rp_frac[i];  // float(i) / float(N)
-    const float bx = mx * fr, by = my * fr;
-    // round() with ties to even, then clamp to the image (imageLoad of a
-    // clamped ivec2)
-    const float iu = rintf((ou - bx) * fW);
-    const float iv = rintf((ovv - by) * fH);
-    const int sx = (int)fminf(fmaxf(iu, 0.0f), maxx);
-    const int sy = (int)fminf(fmaxf(iv, 0.0f), maxy);
-    const int li = sy * W + sx;
+  // taps in batches of kTapBatch: the batch's indices, then all its loads,
+  // then the sum in tap order (the loads of a batch are in flight together
+  // instead of one round trip per tap; the same operations in the same
+  // order as one tap at a time)
+  constexpr int kTapBatch = 8;
+  for (int i0 = 0; i0 < N; i0 += kTapBatch) {
+    int li[kTapBatch];
+#pragma unroll
+    for (int k = 0; k < kTapBatch; ++k) {
+      const float fr = fc.rp_frac[min(i0 + k, N - 1)];  // float(i) / float(N)
+      const float bx = mx * fr, by = my * fr;
+      // round() with ties to even, then clamp to the image (imageLoad of a
+      // clamped ivec2)
+      const float iu = rintf((ou - bx) * fW);
+      const float iv = rintf((ovv - by) * fH);
+      const int sx = (int)fminf(fmaxf(iu, 0.0f), maxx);
+      const int sy = (int)fminf(fmaxf(iv, 0.0f), maxy);
+      li[k] = sy * W + sx;
+    }
     if (INDICES) {
-      ov.reproj_idx[o * (size_t)N + (size_t)i] = li;
+#pragma unroll
+      for (int k = 0; k < kTapBatch; ++k)
+        if (i0 + k < N) ov.reproj_idx[o * (size_t)N + (size_t)(i0 + k)] = li[k];
     } else {
-      const float4 t = ov.prev[li];
-      acc.x += t.x;
-      acc.y += t.y;
-      acc.z += t.z;
-      acc.w += t.w;
+      float4 t[kTapBatch];
+#pragma unroll
+      for (int k = 0; k < kTapBatch; ++k)
+        if (i0 + k < N) t[k] = ov.prev[li[k]];
+#pragma unroll
+      for (int k = 0; k < kTapBatch; ++k) {
+        if (i0 + k < N) {
+          acc.x += t[k].x;
+          acc.y += t[k].y;
+          acc.z += t[k].z;
+          acc.w += t[k].w;
+        }
+      }
     }
   }
   if (!INDICES) {
