@@ -6,7 +6,9 @@
 // list of (view, tile) work items.  A render uploads the views' uniforms,
 // then replays a hipGraph holding [queue reset kernel, persistent march
 // kernel]; the graph is captured once per (tuning, coverage mode, variant,
-// item count).  The persistent grid pulls items from an atomic queue, so
+// item count).  The persistent grid pulls items from 8 atomic lists (each
+// view's tiles dealt to the XCDs in column strips, one list per XCD; an
+// XCD's workgroups take from the other lists once theirs is empty), so
 // uneven views balance themselves; ordering items by descending view cost
 // (steps x rays) keeps the longest work from landing last.
 #include <hip/hip_runtime.h>
@@ -19,7 +21,7 @@
 
 #include "runtime_internal.h"
 
-hipError_t launch_march_batch(const BatchView* views, const BatchItem* items, int n_items, unsigned int* queue,
+hipError_t launch_march_batch(const BatchView* views, const BatchItem* items, const BatchQueues& bq, unsigned int* queue,
                               const uint4* lf, const uint4* hf, const uint4* curl, const uint4* weather,
                               const VolumeViews& vv, float4* color, float4* mask, int coverage_mode, int variant,
                               int lanes_per_ray, int tile_x, int blocks, hipStream_t s);
@@ -37,7 +39,8 @@ struct mtc_batch {
   BatchItem* d_items = nullptr;
   size_t items_cap = 0;
   int n_items = 0;
-  unsigned int* d_queue = nullptr;
+  BatchQueues queues = {};
+  unsigned int* d_queue = nullptr;  // 8 list counters, kBatchQueueStride words apart
   hipStream_t stream = nullptr;
   mtc_batch_tuning tuning = {8, 8, 4, 0};
   std::vector<int> item_steps;  // per-view steps the item order was built for
@@ -93,10 +96,23 @@ int build_items(mtc_batch* b, const mtc_cloud_params* params) {
   if (b->tuning.order == 0) {
     std::stable_sort(order.begin(), order.end(), [&](int a, int c) { return steps[a] > steps[c]; });
   }
-  std::vector<BatchItem> items;
+  // each view's tiles in strip order, tile b to list b % 8 (the XCD that
+  // claims list x runs the view's strips x, x + 8, ...)
+  std::vector<BatchItem> lists[8];
   for (int v : order) {
-    const int tiles = ((b->w[v] + tx - 1) / tx) * ((b->h[v] + ty - 1) / ty);
-    for (int t = 0; t < tiles; ++t) items.push_back(BatchItem{v, t});
+    const int tiles_x = (b->w[v] + tx - 1) / tx, tiles_y = (b->h[v] + ty - 1) / ty;
+    const StripOrder so = strip_order(tiles_x);
+    for (int t = 0; t < tiles_x * tiles_y; ++t) {
+      int sx, sy;
+      strip_tile(t, tiles_x, tiles_y, so.nstrip, so.k, sx, sy);
+      lists[t & 7].push_back(BatchItem{v, sy * tiles_x + sx});
+    }
+  }
+  std::vector<BatchItem> items;
+  for (int x = 0; x < 8; ++x) {
+    b->queues.off[x] = (int)items.size();
+    b->queues.len[x] = (int)lists[x].size();
+    items.insert(items.end(), lists[x].begin(), lists[x].end());
   }
   if (items.size() > b->items_cap) {
     (void)hipFree(b->d_items);
@@ -119,7 +135,7 @@ int capture(mtc_batch* b, int cov) {
   mtc_ctx* c = b->ctx;
   const int variant = c->desc.kernel == MTC_KERNEL_PRECISION ? 1 : 0;
   HIP_OK(hipStreamBeginCapture(b->stream, hipStreamCaptureModeThreadLocal), "batch: begin capture");
-  hipError_t e = launch_march_batch(b->d_views, b->d_items, b->n_items, b->d_queue, c->q_lf, c->q_hf, c->q_curl,
+  hipError_t e = launch_march_batch(b->d_views, b->d_items, b->queues, b->d_queue, c->q_lf, c->q_hf, c->q_curl,
                                     c->q_weather, volume_views(c), b->color, b->mask, cov, variant,
                                     b->tuning.lanes_per_ray, b->tuning.tile_rays_x,
                                     b->tuning.blocks_per_cu * b->cus, b->stream);
@@ -177,7 +193,7 @@ int mtc_batch_create(mtc_ctx* ctx, int32_t n_views, const int32_t* widths, const
   if (e == hipSuccess) e = hipMalloc((void**)&b->mask, off * sizeof(float4));
   if (e == hipSuccess) e = hipMalloc((void**)&b->d_views, (size_t)n_views * sizeof(BatchView));
   if (e == hipSuccess) e = hipHostMalloc((void**)&b->h_views, (size_t)n_views * sizeof(BatchView), hipHostMallocDefault);
-  if (e == hipSuccess) e = hipMalloc((void**)&b->d_queue, 64);
+  if (e == hipSuccess) e = hipMalloc((void**)&b->d_queue, 8 * kBatchQueueStride * sizeof(unsigned int));
   if (e == hipSuccess) e = hipEventCreate(&b->ev0);
   if (e == hipSuccess) e = hipEventCreate(&b->ev1);
   if (e != hipSuccess) {

@@ -1026,34 +1026,6 @@ __device__ __forceinline__ void file_tile(const TileOrder& to, uint32_t tile, ui
   if (pos < to.cap) to.wr[kOrderLevels + (size_t)level * to.cap + pos] = tile;
 }
 
-// Strip order of the fast and precision marches.  Workgroups go to the 8
-// XCDs in turn (workgroup b to XCD (b + c) % 8, c fixed per launch) and
-// each XCD has an L2 of its own.  In raster order every XCD's resident
-// waves are spread over the whole width of the rows being marched, and each
-// L2 holds the texels of all of them.  Here the tiles of each block of k
-// rows are numbered column by column and cut into nstrip equal strips of
-// run = tiles_x * k / nstrip tiles (k is a multiple of nstrip /
-// gcd(tiles_x, nstrip), so run is whole); the workgroups b = x, x + 8,
-// x + 16, ... (one XCD) march strips x, x + 8, ..., x + nstrip - 8 block by
-// block from the top, so that L2 serves an eighth of the area, while each
-// XCD's strips, spread over the width, hold sky, horizon and ocean in the
-// same shares as the others'.  Rows past the last whole block keep raster
-// order.  A bijection on [0, tiles_x * tiles_y) for any nstrip that is a
-// multiple of 8 and such a k; nstrip = 0 keeps raster order.  The order
-// never changes a tile's result.
-__device__ __forceinline__ void strip_tile(int b, int tiles_x, int tiles_y, int nstrip, int k, int& tx, int& ty) {
-  tx = b % tiles_x;
-  ty = b / tiles_x;
-  if (nstrip <= 0 || b >= (tiles_y / k) * tiles_x * k) return;
-  const int run = tiles_x * k / nstrip;
-  const int per = nstrip / 8 * run;  // tiles of one block in one XCD's strips
-  const int j = b >> 3;
-  const int m = j % per;
-  const int idx = ((b & 7) + 8 * (m / run)) * run + m % run;
-  tx = idx / k;
-  ty = (j / per) * k + idx % k;
-}
-
 // One frame: a one-wave workgroup marches an 8 x (8 / G) tile of rays (a
 // finer grain than four-wave groups: the launch drains in smaller steps),
 // in strip order (above).  PAIR: cone rounds two at a time (95 VGPRs, 5
@@ -1180,35 +1152,56 @@ __global__ __launch_bounds__(256) void light_samples_kernel(FrameConsts fc, Fast
 }
 
 // Many views in one launch: a persistent grid pulls (view, tile) work items
-// from a queue (items sorted by estimated cost when the batch was built),
-// kBatchClaim consecutive items per atomic add: every add on the one
-// counter is a device-scope atomic that the memory side serialises, and
-// its round trip is exposed before the tile starts.  Measured on MI355X
-// (96-view set, precision): 1 item per add 18.36 ms, 2 18.04, 4 17.97, 8
-// 17.95.  Each view carries its own FrameConsts, size, output offset and
-// steps.
+// from 8 lists, one per XCD (each view's tiles dealt to the XCDs in strip
+// order, so that an XCD's L2 serves narrow strips of the views; items
+// sorted by estimated cost when the batch was built).  A workgroup takes
+// kBatchClaim consecutive items per atomic add from the list of the XCD it
+// runs on (read from the hardware register), and once that list is empty
+// from the next lists in turn; it remembers the first list that was not
+// empty (counters only grow, so an empty list stays empty) and stops when
+// all 8 are.  Every add is a device-scope atomic that the memory side
+// serialises, and its round trip is exposed before the tile starts;
+// measured on MI355X (96-view set, precision, one list): 1 item per add
+// 18.36 ms, 2 18.04, 4 17.97, 8 17.95.  Each view carries its own
+// FrameConsts, size, output offset and steps.
 constexpr int kBatchClaim = 4;
+constexpr int kHwRegXccId = 20 | (0 << 6) | (3 << 11);  // s_getreg operand: HW_REG_XCC_ID bits [3:0]
 
 template <int COV, bool F16, int G>
 __global__ __launch_bounds__(256) void march_batch_kernel(const BatchView* __restrict__ views,
-                                                          const BatchItem* __restrict__ items, int n_items,
+                                                          const BatchItem* __restrict__ items, BatchQueues bq,
                                                           unsigned int* __restrict__ queue, FastViews fv,
                                                           float4* __restrict__ color, float4* __restrict__ mask,
                                                           int tile_x) {
   __shared__ float sjit[48];
   __shared__ float skern[18];
   __shared__ ConeLds cone_lds[4];
-  __shared__ int s_item;
+  __shared__ int s_item, s_n;
   const int tile_y = 256 / G / tile_x;
-  int it = 0, left = 0;  // next item of the current claim, items left in it
+  const int xcd = __builtin_amdgcn_s_getreg(kHwRegXccId) & 7;
+  int from = 0;               // lists (xcd + from) .. are the ones not yet seen empty
+  int it = 0, left = 0;       // next item of the current claim, items left in it
   for (;;) {
     if (left == 0) {
-      if (threadIdx.x == 0) s_item = (int)atomicAdd(queue, (unsigned)kBatchClaim);
+      if (threadIdx.x == 0) {
+        int got = -1, n = 0;
+        for (; from < 8; ++from) {
+          const int q = (xcd + from) & 7;
+          const unsigned p = atomicAdd(queue + kBatchQueueStride * q, (unsigned)kBatchClaim);
+          if (p < (unsigned)bq.len[q]) {
+            got = bq.off[q] + (int)p;
+            n = min(kBatchClaim, bq.len[q] - (int)p);
+            break;
+          }
+        }
+        s_item = got;
+        s_n = n;
+      }
       __syncthreads();
       it = s_item;
-      left = kBatchClaim;
+      left = s_n;
+      if (it < 0) break;
     }
-    if (it >= n_items) break;
     const BatchItem wi = items[it];
     const BatchView& bv = views[__builtin_amdgcn_readfirstlane(wi.view)];
     if (threadIdx.x < 48) sjit[threadIdx.x] = (&bv.fc.step_jitter[0][0])[threadIdx.x];
@@ -1224,7 +1217,9 @@ __global__ __launch_bounds__(256) void march_batch_kernel(const BatchView* __res
   }
 }
 
-__global__ void reset_queue_kernel(unsigned int* queue) { *queue = 0u; }
+__global__ void reset_queue_kernel(unsigned int* queue) {
+  if (threadIdx.x < 8) queue[kBatchQueueStride * threadIdx.x] = 0u;
+}
 
 // ---------------------------------------------------------------------------
 // footprint packing (run once per volume upload)
@@ -1340,17 +1335,6 @@ hipError_t pack_fast_resources(const VolumeViews& vv, uint4* lf, uint4* hf, uint
   return hipGetLastError();
 }
 
-// Strip order for a launch tiles_x tiles wide: nstrip, a multiple of 8,
-// gives strips about 2.5 tiles (20 rays) wide; k = nstrip / gcd(tiles_x,
-// nstrip) rows per block, the fewest that split evenly.  Measured on
-// MI355X (precision march): 1080p 4x4 frames (60 tiles wide) 24 strips
-// -8% against raster order, 32 -7%, 40 -7%, 48 -6%; 4K 4x4 frames (120
-// tiles wide) 48 strips within 0.3% of raster order, 32 +2%, 64 +0.4%, 128
-// +1.3%.  Three times the rows per block: 1080p +0.7%, 4K -0.7%.
-struct StripOrder {
-  int nstrip, k;
-};
-
 // Launches with at least this many march tiles per compute unit use the
 // single-round kernel.  Measured on MI355X (precision march, tiles per CU:
 // paired-round time -> single-round time): 1080p 4x4 frames (63) 0.2029 ->
@@ -1371,19 +1355,6 @@ static int compute_units() {
   }
   return cached[dev];
 }
-static StripOrder strip_order(int tiles_x) {
-  StripOrder o;
-  o.nstrip = 8 * std::max(1, (tiles_x + 10) / 20);
-  int a = tiles_x, b = o.nstrip;
-  while (b) {
-    const int t = a % b;
-    a = b;
-    b = t;
-  }
-  o.k = o.nstrip / a;  // nstrip / gcd(tiles_x, nstrip)
-  return o;
-}
-
 template <int COV, bool F16, bool FP64, int G, bool BRICK = false>
 static void launch_v(hipStream_t s, const FrameConsts& fc, const mtc::fast::FastViews& fv, const OutputViews& ov,
                      int full, bool with_reproj, const KernelEvents& ke) {
@@ -1527,23 +1498,24 @@ hipError_t launch_light_samples(const FrameConsts& fc, const uint4* lf, const ui
 }
 
 template <int COV, bool F16, int G>
-static void launch_batch_g(const BatchView* views, const BatchItem* items, int n_items, unsigned int* queue,
-                           const mtc::fast::FastViews& fv, float4* color, float4* mask, int tile_x, int blocks,
-                           hipStream_t s) {
-  hipLaunchKernelGGL(mtc::fast::reset_queue_kernel, dim3(1), dim3(1), 0, s, queue);
-  hipLaunchKernelGGL((mtc::fast::march_batch_kernel<COV, F16, G>), dim3(blocks), dim3(256), 0, s, views, items,
-                     n_items, queue, fv, color, mask, tile_x);
+static void launch_batch_g(const BatchView* views, const BatchItem* items, const BatchQueues& bq,
+                           unsigned int* queue, const mtc::fast::FastViews& fv, float4* color, float4* mask,
+                           int tile_x, int blocks, hipStream_t s) {
+  hipLaunchKernelGGL(mtc::fast::reset_queue_kernel, dim3(1), dim3(64), 0, s, queue);
+  hipLaunchKernelGGL((mtc::fast::march_batch_kernel<COV, F16, G>), dim3(blocks), dim3(256), 0, s, views, items, bq,
+                     queue, fv, color, mask, tile_x);
 }
 
-// Enqueue one batched render: queue reset + persistent march over the items.
-// variant 0 = fast, 1 = precision; lanes_per_ray 4 or 8.
-hipError_t launch_march_batch(const BatchView* views, const BatchItem* items, int n_items, unsigned int* queue,
+// Enqueue one batched render: queue reset + persistent march over the items
+// (the 8 lists of bq; queue holds 8 * kBatchQueueStride words).  variant 0 =
+// fast, 1 = precision; lanes_per_ray 4 or 8.
+hipError_t launch_march_batch(const BatchView* views, const BatchItem* items, const BatchQueues& bq, unsigned int* queue,
                               const uint4* lf, const uint4* hf, const uint4* curl, const uint4* weather,
                               const VolumeViews& vv, float4* color, float4* mask, int coverage_mode, int variant,
                               int lanes_per_ray, int tile_x, int blocks, hipStream_t s) {
   mtc::fast::FastViews fv{lf, hf, curl, weather, vv.lf_shift, vv.hf_shift, vv.curl_shift, vv.weather_shift};
   const bool w = coverage_mode == MTC_COVERAGE_WEATHER;
-#define MTC_BATCH(COVV, F16V, GV) launch_batch_g<COVV, F16V, GV>(views, items, n_items, queue, fv, color, mask, tile_x, blocks, s)
+#define MTC_BATCH(COVV, F16V, GV) launch_batch_g<COVV, F16V, GV>(views, items, bq, queue, fv, color, mask, tile_x, blocks, s)
   if (lanes_per_ray == 8) {
     if (variant == 0) { if (w) MTC_BATCH(MTC_COVERAGE_WEATHER, false, 8); else MTC_BATCH(MTC_COVERAGE_CONSTANT, false, 8); }
     else { if (w) MTC_BATCH(MTC_COVERAGE_WEATHER, true, 8); else MTC_BATCH(MTC_COVERAGE_CONSTANT, true, 8); }
