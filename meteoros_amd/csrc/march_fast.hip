@@ -1152,9 +1152,9 @@ __global__ __launch_bounds__(256) void light_samples_kernel(FrameConsts fc, Fast
 }
 
 // Many views in one launch: a persistent grid pulls (view, tile) work items
-// from 8 lists, one per XCD (each view's tiles dealt to the XCDs in strip
-// order, so that an XCD's L2 serves narrow strips of the views; items
-// sorted by estimated cost when the batch was built).  A workgroup takes
+// from 8 lists, one per XCD (each view's tiles dealt to the XCDs in column
+// strips, so that an XCD's L2 serves one eighth of each view; items sorted
+// by estimated cost when the batch was built).  A workgroup takes
 // kBatchClaim consecutive items per atomic add from the list of the XCD it
 // runs on (read from the hardware register), and once that list is empty
 // from the next lists in turn; it remembers the first list that was not

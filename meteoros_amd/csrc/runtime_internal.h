@@ -179,8 +179,9 @@ struct BatchItem {
   int tile;
 };
 // The batch's items are kept in 8 lists, list x holding each view's tiles
-// b = x, x + 8, ... of the strip order; a workgroup takes items from the
-// list of the XCD it runs on, then from the others.  List x is
+// b = x, x + 8, ... of the strip order with 8 strips (one strip of each view
+// per list); a workgroup takes items from the list of the XCD it runs on,
+// then from the others.  List x is
 // items[off[x], off[x] + len[x]); its counter is word kBatchQueueStride * x
 // of the queue buffer (one 128-byte line each).
 constexpr int kBatchQueueStride = 32;
