@@ -234,10 +234,34 @@ __global__ __launch_bounds__(256) void god_rays_kernel(PostConsts pc, const floa
   }
 }
 
+// a / b through the hardware reciprocal (about 1 ulp) and one FMA
+// correction: q = a y, r = a - q b exactly, q + r y.  Four VALU operations
+// instead of the IEEE division's dozen; the result is the correctly rounded
+// quotient except in rare cases, where it is one ulp off.  For finite,
+// normal, non-zero b (every denominator of the tone map and TXAA passes:
+// image sizes, the Uncharted-2 curve's x (A x + B) + D F >= 0.06, lengths,
+// 2A of the shell quadratic, the projected depth, the clip box's half
+// extents (>= 1e-10) and the luminance floor (>= 0.2)).
+struct Recip {
+  float b, y;
+};
+__device__ __forceinline__ Recip recip(float b) { return Recip{b, __builtin_amdgcn_rcpf(b)}; }
+__device__ __forceinline__ float divr(float a, const Recip& d) {
+  const float q = a * d.y;
+  const float r = __builtin_fmaf(-q, d.b, a);
+  return __builtin_fmaf(r, d.y, q);
+}
+__device__ __forceinline__ float divr(float a, float b) { return divr(a, recip(b)); }
+__device__ __forceinline__ f3 divr(f3 a, const Recip& d) { return mk(divr(a.x, d), divr(a.y, d), divr(a.z, d)); }
+__device__ __forceinline__ f3 normalize_r(f3 a) { return divr(a, recip(length(a))); }
+__device__ __forceinline__ float4 f4divr(float4 a, const Recip& d) {
+  return make_float4(divr(a.x, d), divr(a.y, d), divr(a.z, d), divr(a.w, d));
+}
+
 // ((x (A x + C B) + D E) / (x (A x + B) + D F)) - E / F
 __device__ __forceinline__ float uncharted2(float x) {
   const float A = 0.15f, B = 0.50f, C = 0.10f, D = 0.20f, E = 0.02f, F = 0.30f;
-  return ((x * (A * x + C * B) + D * E) / (x * (A * x + B) + D * F)) - E / F;
+  return divr(x * (A * x + C * B) + D * E, x * (A * x + B) + D * F) - E / F;
 }
 
 __device__ __forceinline__ uint32_t wang_hash(uint32_t u, uint32_t v, uint32_t s) {
@@ -261,7 +285,7 @@ __global__ __launch_bounds__(256) void tone_map_kernel(const float4* __restrict_
   const int px = blockIdx.x * 64 + (threadIdx.x & 63);
   const int py = blockIdx.y * 4 + (threadIdx.x >> 6);
   if (px >= W || py >= H) return;
-  const float u = ((float)px + 0.5f) / (float)W, v = ((float)py + 0.5f) / (float)H;
+  const float u = divr((float)px + 0.5f, (float)W), v = divr((float)py + 0.5f, (float)H);
   const float x = u * (float)W - 0.5f, y = v * (float)H - 0.5f;
   const float fx = floorf(x), fy = floorf(y);
   const float wx = x - fx, wy = y - fy;
@@ -309,29 +333,6 @@ __device__ __forceinline__ float4 f4add(float4 a, float4 b) {
   return make_float4(a.x + b.x, a.y + b.y, a.z + b.z, a.w + b.w);
 }
 __device__ __forceinline__ float4 f4scale(float4 a, float s) { return make_float4(a.x * s, a.y * s, a.z * s, a.w * s); }
-
-// a / b through the hardware reciprocal (about 1 ulp) and one FMA
-// correction: q = a y, r = a - q b exactly, q + r y.  Four VALU operations
-// instead of the IEEE division's dozen; the result is the correctly rounded
-// quotient except in rare cases, where it is one ulp off.  For finite,
-// normal, non-zero b (every denominator of the TXAA pass: image sizes,
-// lengths, 2A of the shell quadratic, the projected depth, the clip box's
-// half extents (>= 1e-10) and the luminance floor (>= 0.2)).
-struct Recip {
-  float b, y;
-};
-__device__ __forceinline__ Recip recip(float b) { return Recip{b, __builtin_amdgcn_rcpf(b)}; }
-__device__ __forceinline__ float divr(float a, const Recip& d) {
-  const float q = a * d.y;
-  const float r = __builtin_fmaf(-q, d.b, a);
-  return __builtin_fmaf(r, d.y, q);
-}
-__device__ __forceinline__ float divr(float a, float b) { return divr(a, recip(b)); }
-__device__ __forceinline__ f3 divr(f3 a, const Recip& d) { return mk(divr(a.x, d), divr(a.y, d), divr(a.z, d)); }
-__device__ __forceinline__ f3 normalize_r(f3 a) { return divr(a, recip(length(a))); }
-__device__ __forceinline__ float4 f4divr(float4 a, const Recip& d) {
-  return make_float4(divr(a.x, d), divr(a.y, d), divr(a.z, d), divr(a.w, d));
-}
 
 // imageLoad of the current RGBA8 frame; outside the image it returns 0
 __device__ __forceinline__ float4 load_cur(const uint32_t* __restrict__ img, int W, int H, int x, int y) {
