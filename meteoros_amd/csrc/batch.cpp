@@ -97,7 +97,7 @@ int build_items(mtc_batch* b, const mtc_cloud_params* params) {
     std::stable_sort(order.begin(), order.end(), [&](int a, int c) { return steps[a] > steps[c]; });
   }
   // each view's tiles in strip order, tile b to list b % 8 (the XCD that
-  // claims list x runs the view's strips x, x + 8, ...)
+  // claims list x runs column strip x of every view)
   std::vector<BatchItem> lists[8];
   for (int v : order) {
     const int tiles_x = (b->w[v] + tx - 1) / tx, tiles_y = (b->h[v] + ty - 1) / ty;
