@@ -94,8 +94,9 @@ inline void launch_timed(K kernel, dim3 grid, dim3 block, hipStream_t s, const K
 // mapping is always a permutation; otherwise (first frame, full frames)
 // workgroup r marches tile r.  The order only changes which workgroup
 // marches which tile when, never a result.  (The fast and precision
-// marches keep raster order: for them, resident waves scattered over the
-// image lose more texel-cache locality than the shorter tail gains.)
+// marches deal their tiles in the strip order below instead: for them,
+// resident waves scattered over the image lose more texel-cache locality
+// than the shorter tail gains.)
 constexpr int kOrderLevels = 16;
 struct TileOrder {
   const uint32_t* rd = nullptr;  // [kOrderLevels counts][kOrderLevels x cap tiles]
