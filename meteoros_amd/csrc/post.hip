@@ -110,19 +110,29 @@ __global__ __launch_bounds__(256) void god_rays_setup_kernel(PostConsts pc, cons
 // horizontal position once for its four pixels, and at sample i the taps
 // of the whole workgroup lie in one region of the decoded map: at most
 // kGodRayStageRows rows (from the table's header) x kGodRayStageCols
-// columns (from the first lane's column, aligned down to 4).  The workgroup
-// stages that region into LDS with 16-byte loads (324 of them, two per
-// lane at most) and every tap reads its four texels from LDS.  Staging runs
-// two samples ahead in registers and one ahead in the second LDS buffer,
-// the table entries one sample ahead in scalar registers; one barrier per
-// sample.  Per tap: two ds_read2_b32 and six FMA-class operations, the
-// same arithmetic as a per-pixel bilinear tap.
+// columns (from the first lane's column less one, aligned down to 4), and fewer
+// further along the walk, where the samples of adjacent pixels close up.
+// The workgroup stages the region of the sample's phase (kGodRayPhaseRows x
+// kGodRayPhaseBlocks 16-byte loads: 324, 196, 100, 42; lane t loads slot t
+// and, in the first phase, t + 256) into LDS, and every tap reads its four
+// texels from LDS.  Staging runs two samples ahead in registers and one
+// ahead in the second LDS buffer, the table entries one sample ahead in
+// scalar registers; one barrier per sample.  Per tap: two ds_read2_b32 and
+// six FMA-class operations, the same arithmetic as a per-pixel bilinear tap.
+template <int ROWS, int BLOCKS>
+struct GodRayShape {
+  static constexpr int slots = ROWS * BLOCKS;
+  static constexpr int blocks = BLOCKS;
+  static_assert(ROWS <= kGodRayStageRows && 4 * BLOCKS <= kGodRayStageCols && slots <= 512,
+                "a phase's region fits the LDS buffer and two loads per lane");
+};
+template <int P>
+using GodRayPhase = GodRayShape<kGodRayPhaseRows[P], kGodRayPhaseBlocks[P]>;
+
 __global__ __launch_bounds__(256) void god_rays_kernel(PostConsts pc, const float* __restrict__ map,
                                                        const GodRayEntry* __restrict__ tab, float4* __restrict__ img) {
   constexpr int REGION = kGodRayStageRows * kGodRayStageCols;
-  constexpr int BLOCKS = kGodRayStageCols / 4;
-  constexpr int SLOTS = kGodRayStageRows * BLOCKS;
-  static_assert(SLOTS > 256 && SLOTS <= 512, "two staging loads per lane");
+  static_assert(kGodRaySamples == 4 * kGodRayPhaseSamples && kGodRayPhaseSamples >= 3, "four phases");
   __shared__ __attribute__((aligned(16))) float stage[2 * REGION];
   typedef float f4v __attribute__((ext_vector_type(4)));
   typedef const __attribute__((address_space(1))) char gchar;
@@ -138,51 +148,72 @@ __global__ __launch_bounds__(256) void god_rays_kernel(PostConsts pc, const floa
   const float du = ((u - pc.sun_ss[0]) / (float)kGodRaySamples) * 1.0f;
   const float Wf = (float)W;
   const GodRayEntry* __restrict__ gt = tab + (size_t)g * kGodRaySamples * kGodRayTableStride;
-  // this lane's staging slots (row, 4-column block) in the map and in LDS
-  const int t0 = tid, t1 = tid + 256;
-  const bool has1 = t1 < SLOTS;
-  const uint32_t goff0 = (uint32_t)((t0 / BLOCKS) * P + 4 * (t0 % BLOCKS)) * 4u;
-  const uint32_t goff1 = has1 ? (uint32_t)((t1 / BLOCKS) * P + 4 * (t1 % BLOCKS)) * 4u : 0u;
-  const uint32_t loff0 = (uint32_t)((t0 / BLOCKS) * kGodRayStageCols + 4 * (t0 % BLOCKS)) * 4u;
-  const uint32_t loff1 = has1 ? (uint32_t)((t1 / BLOCKS) * kGodRayStageCols + 4 * (t1 % BLOCKS)) * 4u : 0u;
+  // this lane's staging slots (row, 4-column block) of a phase's region, in
+  // the map (goff*, for the samples being loaded) and in LDS (loff*, for the
+  // samples being stored), set when the phase changes
+  uint32_t goff0 = 0, goff1 = 0, loff0 = 0, loff1 = 0;
+  auto map_slots = [&](auto S) {
+    using Sh = decltype(S);
+    goff0 = (uint32_t)((tid / Sh::blocks) * P + 4 * (tid % Sh::blocks)) * 4u;
+    if (Sh::slots > 256) goff1 = (uint32_t)(((tid + 256) / Sh::blocks) * P + 4 * ((tid + 256) % Sh::blocks)) * 4u;
+  };
+  auto lds_slots = [&](auto S) {
+    using Sh = decltype(S);
+    loff0 = (uint32_t)((tid / Sh::blocks) * kGodRayStageCols + 4 * (tid % Sh::blocks)) * 4u;
+    if (Sh::slots > 256)
+      loff1 = (uint32_t)(((tid + 256) / Sh::blocks) * kGodRayStageCols + 4 * ((tid + 256) % Sh::blocks)) * 4u;
+  };
   // sample position: x = u * W - 0.5 (sampler), weight and column; the
-  // region's first column, 4-aligned, from the lowest lane's column
+  // region's first stored column: one before the first lane's, aligned down
+  // to 4.  The lanes' columns grow with the lane in exact arithmetic; in the
+  // FP32 walk a lane can land one column below the first lane's where the
+  // samples lie a hundredth of a texel apart (seen at widths of 8192 and
+  // more), never further, so the region starts one column early.
   auto advance = [&](float& wx, int& ix, int& cs) {
     const float x = u * Wf - 0.5f;
     const float fx = floorf(x);
     wx = x - fx;
     // the walk stays between pixel and sun: floor lies in [-1, W - 1]
     ix = min(max((int)fx, -1), W - 1);
-    cs = (__builtin_amdgcn_readfirstlane(ix) + 4) & ~3;
+    cs = (__builtin_amdgcn_readfirstlane(ix) + 3) & ~3;
     u -= du;
   };
+  f4v d0, d1;
   // the region's 16-byte blocks of the sample whose first map row is `row`
-  auto load_region = [&](uint32_t row, int cs, f4v& d0, f4v& d1) {
+  auto load_region = [&](auto S, uint32_t row, int cs) {
+    using Sh = decltype(S);
     const uint64_t rb = (uint64_t)map + ((uint64_t)row * (uint64_t)P + (uint64_t)cs) * sizeof(float);
     // the region's address is uniform: kept whole in scalar registers (each
     // half through uint32_t, since readfirstlane returns int)
     const uint32_t hi = (uint32_t)__builtin_amdgcn_readfirstlane((int)(uint32_t)(rb >> 32));
     const uint32_t lo = (uint32_t)__builtin_amdgcn_readfirstlane((int)(uint32_t)rb);
     gchar* b = (gchar*)(((uint64_t)hi << 32) | (uint64_t)lo);
-    d0 = *(const __attribute__((address_space(1))) f4v*)(b + goff0);
-    if (has1) d1 = *(const __attribute__((address_space(1))) f4v*)(b + goff1);
+    if (Sh::slots >= 256 || tid < Sh::slots) d0 = *(const __attribute__((address_space(1))) f4v*)(b + goff0);
+    if (Sh::slots > 256 && tid + 256 < Sh::slots) d1 = *(const __attribute__((address_space(1))) f4v*)(b + goff1);
+  };
+  auto store_region = [&](auto S, char* wb) {
+    using Sh = decltype(S);
+    if (Sh::slots >= 256 || tid < Sh::slots) *(f4v*)(wb + loff0) = d0;
+    if (Sh::slots > 256 && tid + 256 < Sh::slots) *(f4v*)(wb + loff1) = d1;
   };
   float wx_c, wx_n;
   int ix_c, ix_n, cs_c, cs_n;
-  f4v d0, d1;
+  map_slots(GodRayPhase<0>{});
+  lds_slots(GodRayPhase<0>{});
   advance(wx_c, ix_c, cs_c);
-  load_region(gt[0].a, cs_c, d0, d1);
-  *(f4v*)((char*)stage + loff0) = d0;
-  if (has1) *(f4v*)((char*)stage + loff1) = d1;
+  load_region(GodRayPhase<0>{}, gt[0].a, cs_c);
+  store_region(GodRayPhase<0>{}, (char*)stage);
   advance(wx_n, ix_n, cs_n);
-  load_region(gt[kGodRayTableStride].a, cs_n, d0, d1);
+  load_region(GodRayPhase<0>{}, gt[kGodRayTableStride].a, cs_n);
   GodRayEntry nx[4];
 #pragma unroll
   for (int k = 0; k < 4; ++k) nx[k] = gt[1 + 4 * w + k];
   uint32_t row2 = gt[2 * kGodRayTableStride].a;
   __syncthreads();
   float acc[4] = {0.0f, 0.0f, 0.0f, 0.0f};
-  for (int i = 0; i < kGodRaySamples; ++i) {
+  // sample i: its taps from stage[i & 1], sample i + 1's region (phase of
+  // SW) into the other buffer, sample i + 2's region (phase of SL) loaded
+  auto sample = [&](auto SW, auto SL, int i) {
     GodRayEntry cur[4];
 #pragma unroll
     for (int k = 0; k < 4; ++k) cur[k] = nx[k];
@@ -192,7 +223,6 @@ __global__ __launch_bounds__(256) void god_rays_kernel(PostConsts pc, const floa
       for (int k = 0; k < 4; ++k) nx[k] = gt[(size_t)(i + 1) * kGodRayTableStride + 1 + 4 * w + k];
     }
     if (i + 3 < kGodRaySamples) row2 = gt[(size_t)(i + 3) * kGodRayTableStride].a;
-    // sample i from stage[i & 1]
     const char* sb = (const char*)(stage + (i & 1) * REGION) + (uint32_t)(ix_c + 4 - cs_c) * 4u;
 #pragma unroll
     for (int k = 0; k < 4; ++k) {
@@ -203,21 +233,44 @@ __global__ __launch_bounds__(256) void god_rays_kernel(PostConsts pc, const floa
       // colour and WEIGHT_FOR_SAMPLE applied once below
       acc[k] = __fmaf_rn(cur[k].w1, b, __fmaf_rn(cur[k].w0, t, acc[k]));
     }
-    // sample i + 1's region (loaded one iteration ago) into the other buffer
-    if (i + 1 < kGodRaySamples) {
-      char* wb = (char*)(stage + ((i + 1) & 1) * REGION);
-      *(f4v*)(wb + loff0) = d0;
-      if (has1) *(f4v*)(wb + loff1) = d1;
-    }
+    if (i + 1 < kGodRaySamples) store_region(SW, (char*)(stage + ((i + 1) & 1) * REGION));
     wx_c = wx_n;
     ix_c = ix_n;
     cs_c = cs_n;
     if (i + 2 < kGodRaySamples) {
       advance(wx_n, ix_n, cs_n);
-      load_region(row, cs_n, d0, d1);
+      load_region(SL, row, cs_n);
     }
     __syncthreads();
-  }
+  };
+  // Phase p covers samples [25 p, 25 p + 25).  Its loads start two samples
+  // early (iteration 25 p - 2) and its stores one sample early, so each
+  // phase change is one iteration of its own.  Every run of iterations,
+  // including those single ones, is a loop unrolled by two: 110 us at 1080p
+  // against 121.5 us not unrolled, in 62 VGPRs (eight waves per SIMD; with
+  // the single iterations written as plain calls the kernel takes 68).
+  constexpr int L = kGodRayPhaseSamples;
+  int i = 0;
+#pragma unroll 2
+  for (; i < L - 2; ++i) sample(GodRayPhase<0>{}, GodRayPhase<0>{}, i);
+  map_slots(GodRayPhase<1>{});
+#pragma unroll 2
+  for (; i < L - 1; ++i) sample(GodRayPhase<0>{}, GodRayPhase<1>{}, i);
+  lds_slots(GodRayPhase<1>{});
+#pragma unroll 2
+  for (; i < 2 * L - 2; ++i) sample(GodRayPhase<1>{}, GodRayPhase<1>{}, i);
+  map_slots(GodRayPhase<2>{});
+#pragma unroll 2
+  for (; i < 2 * L - 1; ++i) sample(GodRayPhase<1>{}, GodRayPhase<2>{}, i);
+  lds_slots(GodRayPhase<2>{});
+#pragma unroll 2
+  for (; i < 3 * L - 2; ++i) sample(GodRayPhase<2>{}, GodRayPhase<2>{}, i);
+  map_slots(GodRayPhase<3>{});
+#pragma unroll 2
+  for (; i < 3 * L - 1; ++i) sample(GodRayPhase<2>{}, GodRayPhase<3>{}, i);
+  lds_slots(GodRayPhase<3>{});
+#pragma unroll 2
+  for (; i < kGodRaySamples; ++i) sample(GodRayPhase<3>{}, GodRayPhase<3>{}, i);
   if (px >= W) return;
 #pragma unroll
   for (int k = 0; k < 4; ++k) {

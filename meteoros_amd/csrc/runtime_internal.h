@@ -224,6 +224,16 @@ constexpr int kGodRayStageRows = kGodRayGroupRows + 2;  // 16 rows at spacing <=
 constexpr int kGodRayStageCols = 72;                   // 64 lanes at spacing <= 1 + 4-aligned start + 1
 constexpr int kGodRayTableStride = kGodRayGroupRows + 1;  // entries per (row group, sample)
 
+// The staged region shrinks along the walk.  Sample i of adjacent pixels
+// lies (1 - i / 100) texels apart, so from sample 25 p on (phase p) the 16
+// rows and 64 columns of a workgroup span at most ceil(15 s) + 2 texel rows
+// and ceil(63 s) + 5 columns (4-aligned start), s = 1 - p / 4, with at
+// least half a texel to spare for the walk's FP32 rounding.  Rows x 4-column
+// blocks staged per phase (the LDS row stride stays kGodRayStageCols):
+constexpr int kGodRayPhaseSamples = kGodRaySamples / 4;
+constexpr int kGodRayPhaseRows[4] = {18, 14, 10, 7};
+constexpr int kGodRayPhaseBlocks[4] = {18, 14, 10, 6};
+
 // One entry of the blur's table, per (row group, sample): entry 0 holds in
 // `a` the first decoded-map row staged for the sample; entry 1 + k, for
 // pixel row k of the group, the byte offset in the staged region of the
