@@ -387,10 +387,19 @@ __device__ __forceinline__ float4 f4add(float4 a, float4 b) {
 }
 __device__ __forceinline__ float4 f4scale(float4 a, float s) { return make_float4(a.x * s, a.y * s, a.z * s, a.w * s); }
 
+// Texel (x, y) of a W x H RGBA8 image.  Images hold at most 16384^2 texels,
+// so the byte offset fits 32 bits: one 32-bit multiply-add per texel and a
+// load addressed from the image's base in scalar registers, instead of
+// 64-bit address arithmetic per tap.
+__device__ __forceinline__ uint32_t texel8(const uint32_t* __restrict__ img, int W, int x, int y) {
+  const uint32_t off = ((uint32_t)y * (uint32_t)W + (uint32_t)x) * 4u;
+  return *(const uint32_t*)((const char*)img + off);
+}
+
 // imageLoad of the current RGBA8 frame; outside the image it returns 0
 __device__ __forceinline__ float4 load_cur(const uint32_t* __restrict__ img, int W, int H, int x, int y) {
   if (x < 0 || y < 0 || x >= W || y >= H) return make_float4(0.0f, 0.0f, 0.0f, 0.0f);
-  return unorm4(img[(size_t)y * W + x]);
+  return unorm4(texel8(img, W, x, y));
 }
 
 // texture(prevFrameImage, uv): RGBA8 codes filtered, then scaled by 1/255;
@@ -404,8 +413,7 @@ __device__ __forceinline__ float4 sample_prev(const uint32_t* __restrict__ img, 
 #pragma unroll
   for (int k = 0; k < 4; ++k) {
     const int xx = ix + (k & 1), yy = iy + (k >> 1);
-    t[k] = (xx < 0 || yy < 0 || xx >= W || yy >= H) ? c4{0.0f, 0.0f, 0.0f, 255.0f}
-                                                    : unpack8(img[(size_t)yy * W + xx]);
+    t[k] = (xx < 0 || yy < 0 || xx >= W || yy >= H) ? c4{0.0f, 0.0f, 0.0f, 255.0f} : unpack8(texel8(img, W, xx, yy));
   }
   c4 r = scale4(lerp4(lerp4(t[0], t[1], wx), lerp4(t[2], t[3], wx), wy), MTC_INV255);
   return make_float4(r.r, r.g, r.b, r.a);
@@ -415,7 +423,8 @@ __global__ __launch_bounds__(256) void txaa_kernel(TxaaConsts tc, const uint32_t
                                                    const uint32_t* __restrict__ prev, uint32_t* __restrict__ out) {
   const int W = tc.width, H = tc.height;
   const int px = blockIdx.x * 64 + (threadIdx.x & 63);
-  const int py = blockIdx.y * 4 + (threadIdx.x >> 6);
+  // one pixel row per wave: the row index is uniform (scalar arithmetic)
+  const int py = blockIdx.y * 4 + __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
   if (px >= W || py >= H) return;
   const float u = divr((float)px + 0.5f, (float)W), v = divr((float)py + 0.5f, (float)H);
   // castRay (:87-116) with the basis folded per frame
@@ -447,12 +456,38 @@ __global__ __launch_bounds__(256) void txaa_kernel(TxaaConsts tc, const uint32_t
   const float nz = -od.z;
   od = divr(od, recip(nz));
   const float ou = divr(od.x, tc.tfx) * 0.5f + 0.5f, ov = divr(od.y, tc.tfy) * 0.5f + 0.5f;
-  // neighbourHoodClamping (:191-220): 3x3 and cross-5 min / max / mean
-  const float4 ctl = load_cur(cur, W, H, px - 1, py - 1), ctc = load_cur(cur, W, H, px, py - 1),
-               ctr = load_cur(cur, W, H, px + 1, py - 1), cml = load_cur(cur, W, H, px - 1, py),
-               cmc = load_cur(cur, W, H, px, py), cmr = load_cur(cur, W, H, px + 1, py),
-               cbl = load_cur(cur, W, H, px - 1, py + 1), cbc = load_cur(cur, W, H, px, py + 1),
-               cbr = load_cur(cur, W, H, px + 1, py + 1);
+  // neighbourHoodClamping (:191-220): 3x3 and cross-5 min / max / mean.  A
+  // wave is one pixel row of 64 columns; when its row and columns and their
+  // neighbours all lie inside the image (every wave but those on the
+  // border), the nine texels are loaded without the per-tap bounds tests.
+  float4 ctl, ctc, ctr, cml, cmc, cmr, cbl, cbc, cbr;
+  const int x0 = blockIdx.x * 64;
+  if (x0 >= 1 && x0 + 64 < W && py >= 1 && py + 1 < H) {
+    // byte offsets of the centre column in the three rows; the left and
+    // right texels are the loads' immediate offsets -4 and +4
+    const uint32_t rs = (uint32_t)W * 4u;
+    const uint32_t om = ((uint32_t)py * (uint32_t)W + (uint32_t)px) * 4u, ot = om - rs, ob = om + rs;
+    const char* c8 = (const char*)cur;
+    ctl = unorm4(*(const uint32_t*)(c8 + ot - 4));
+    ctc = unorm4(*(const uint32_t*)(c8 + ot));
+    ctr = unorm4(*(const uint32_t*)(c8 + ot + 4));
+    cml = unorm4(*(const uint32_t*)(c8 + om - 4));
+    cmc = unorm4(*(const uint32_t*)(c8 + om));
+    cmr = unorm4(*(const uint32_t*)(c8 + om + 4));
+    cbl = unorm4(*(const uint32_t*)(c8 + ob - 4));
+    cbc = unorm4(*(const uint32_t*)(c8 + ob));
+    cbr = unorm4(*(const uint32_t*)(c8 + ob + 4));
+  } else {
+    ctl = load_cur(cur, W, H, px - 1, py - 1);
+    ctc = load_cur(cur, W, H, px, py - 1);
+    ctr = load_cur(cur, W, H, px + 1, py - 1);
+    cml = load_cur(cur, W, H, px - 1, py);
+    cmc = load_cur(cur, W, H, px, py);
+    cmr = load_cur(cur, W, H, px + 1, py);
+    cbl = load_cur(cur, W, H, px - 1, py + 1);
+    cbc = load_cur(cur, W, H, px, py + 1);
+    cbr = load_cur(cur, W, H, px + 1, py + 1);
+  }
   float4 cmin = f4min(ctl, f4min(ctc, f4min(ctr, f4min(cml, f4min(cmc, f4min(cmr, f4min(cbl, f4min(cbc, cbr))))))));
   float4 cmax = f4max(ctl, f4max(ctc, f4max(ctr, f4max(cml, f4max(cmc, f4max(cmr, f4max(cbl, f4max(cbc, cbr))))))));
   float4 cavg = f4divr(f4add(f4add(f4add(f4add(f4add(f4add(f4add(f4add(ctl, ctc), ctr), cml), cmc), cmr), cbl), cbc),
