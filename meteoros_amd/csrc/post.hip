@@ -336,15 +336,36 @@ template <bool F32OUT>
 __global__ __launch_bounds__(256) void tone_map_kernel(const float4* __restrict__ img, int W, int H, uint32_t seed_s,
                                                        float whitemap, void* __restrict__ out) {
   const int px = blockIdx.x * 64 + (threadIdx.x & 63);
-  const int py = blockIdx.y * 4 + (threadIdx.x >> 6);
+  // one pixel row per wave: the row index is uniform (scalar arithmetic)
+  const int py = blockIdx.y * 4 + __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
   if (px >= W || py >= H) return;
   const float u = divr((float)px + 0.5f, (float)W), v = divr((float)py + 0.5f, (float)H);
   const float x = u * (float)W - 0.5f, y = v * (float)H - 0.5f;
   const float fx = floorf(x), fy = floorf(y);
   const float wx = x - fx, wy = y - fy;
   const int ix = (int)fx, iy = (int)fy;
-  const float4 t00 = texel_or_border(img, W, H, ix, iy), t10 = texel_or_border(img, W, H, ix + 1, iy);
-  const float4 t01 = texel_or_border(img, W, H, ix, iy + 1), t11 = texel_or_border(img, W, H, ix + 1, iy + 1);
+  // At a pixel centre x and y lie within a rounding error of px and py, so
+  // the taps are columns px - 1 .. px + 1 and rows py - 1 .. py + 1.  A wave
+  // whose row and columns and their neighbours lie inside the image reads
+  // them without bounds tests: its texel rows are uniform (v depends on py
+  // only), so each tap is a row pointer in scalar registers plus a 32-bit
+  // column offset.
+  float4 t00, t10, t01, t11;
+  const int x0 = blockIdx.x * 64;
+  if (x0 >= 1 && x0 + 64 < W && py >= 1 && py + 1 < H) {
+    const float4* r0 = img + (size_t)__builtin_amdgcn_readfirstlane(iy) * W;
+    const char* c0 = (const char*)r0 + (uint32_t)ix * 16u;
+    const char* c1 = (const char*)(r0 + W) + (uint32_t)ix * 16u;
+    t00 = *(const float4*)c0;
+    t10 = *(const float4*)(c0 + 16);
+    t01 = *(const float4*)c1;
+    t11 = *(const float4*)(c1 + 16);
+  } else {
+    t00 = texel_or_border(img, W, H, ix, iy);
+    t10 = texel_or_border(img, W, H, ix + 1, iy);
+    t01 = texel_or_border(img, W, H, ix, iy + 1);
+    t11 = texel_or_border(img, W, H, ix + 1, iy + 1);
+  }
   float rgb[3] = {lerpf(lerpf(t00.x, t10.x, wx), lerpf(t01.x, t11.x, wx), wy),
                   lerpf(lerpf(t00.y, t10.y, wx), lerpf(t01.y, t11.y, wx), wy),
                   lerpf(lerpf(t00.z, t10.z, wx), lerpf(t01.z, t11.z, wx), wy)};
@@ -364,7 +385,9 @@ __global__ __launch_bounds__(256) void tone_map_kernel(const float4* __restrict_
   if (F32OUT) {
     ((float4*)out)[i] = make_float4(o[0], o[1], o[2], 1.0f);
   } else {
-    ((uint32_t*)out)[i] = to_unorm8(o[0]) | (to_unorm8(o[1]) << 8) | (to_unorm8(o[2]) << 16) | (255u << 24);
+    // W x H <= 16384^2 RGBA8 texels: the byte offset fits 32 bits
+    *(uint32_t*)((char*)out + (uint32_t)i * 4u) =
+        to_unorm8(o[0]) | (to_unorm8(o[1]) << 8) | (to_unorm8(o[2]) << 16) | (255u << 24);
   }
 }
 
