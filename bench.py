@@ -305,7 +305,8 @@ def run_batch(kernel: int, renders: int = 5, check_rays: int = 0) -> dict:
 # (profiles/god_rays_1080p_counters.md, profiles/march_kernels_1080p_kernel_stats.md):
 # only the tone map is priced by its bytes.
 POST_BYTES_PER_PX = {"god_rays": 56, "tone_map": 20, "txaa": 12}
-POST_BOUND = {"god_rays": "texture data path (LDS staging loads) and VALU, not HBM", "tone_map": "hbm",
+POST_BOUND = {"god_rays": "texture data path (LDS staging loads) and VALU, not HBM",
+              "tone_map": "hbm and valu (41 MB per pass; ~125 VALU instructions and three log / exp pairs per pixel)",
               "txaa": "valu (ray, shell hit and re-projection per pixel; 9-tap neighbourhood box and clip)"}
 
 
