@@ -227,9 +227,10 @@ constexpr int kGodRayTableStride = kGodRayGroupRows + 1;  // entries per (row gr
 // The staged region shrinks along the walk.  Sample i of adjacent pixels
 // lies (1 - i / 100) texels apart, so from sample 25 p on (phase p) the 16
 // rows and 64 columns of a workgroup span at most ceil(15 s) + 2 texel rows
-// and ceil(63 s) + 5 columns (4-aligned start), s = 1 - p / 4, with at
-// least half a texel to spare for the walk's FP32 rounding.  Rows x 4-column
-// blocks staged per phase (the LDS row stride stays kGodRayStageCols):
+// and ceil(63 s) + 6 columns (the region starts one column before the first
+// lane's, aligned down to 4), s = 1 - p / 4, with at least half a texel to
+// spare for the walk's FP32 rounding.  Rows x 4-column blocks staged per
+// phase (the LDS row stride stays kGodRayStageCols):
 constexpr int kGodRayPhaseSamples = kGodRaySamples / 4;
 constexpr int kGodRayPhaseRows[4] = {18, 14, 10, 7};
 constexpr int kGodRayPhaseBlocks[4] = {18, 14, 10, 6};
