@@ -145,6 +145,44 @@ def test_txaa_matches_oracle(yaw):
             old = cur
 
 
+@pytest.mark.parametrize("W,H", [(65, 37), (130, 17), (3, 2), (200, 300)])
+def test_tone_map_and_txaa_odd_sizes_match_oracle(W, H):
+    """Both passes read neighbouring texels without bounds tests in waves
+    (one pixel row of 64 columns) whose row, columns and neighbours lie
+    inside the image, and with them elsewhere: 65 x 37 and 3 x 2 have only
+    bounds-tested waves, 130 x 17 and 200 x 300 both kinds, with partial
+    waves at the right edge."""
+    params, sun = M.preset_params(2)
+    camera = ground_camera(W, H, 25.0)
+    v = volumes(*DIMS)
+    with M.CloudStage(W, H, M.VolumeSpec(*DIMS), kernel=A.MTC_KERNEL_FAST) as st:
+        st.set_volumes(v.lf, v.hf, v.curl, v.weather)
+        history = np.zeros((H, W, 4), np.uint8)
+        old = camera.ubo()
+        for k in range(2):
+            camera.rotate_about_up(0.5)
+            cur = camera.ubo()
+            t = M.Time()
+            t.ubo.frame_count_mod16 = 4 + k
+            t.ubo.time[1] = 2.0 + k / 60.0
+            st.march_full(cur, t.ubo, sun, params)
+            rgba, _ = st.read_output()
+            st.tone_map(t.ubo)
+            disp = st.read_display()
+            _, disp_ref = O.tone_map(int(t.ubo.time[1]), rgba, precision=32)
+            d = np.abs(disp.astype(np.int16) - disp_ref.astype(np.int16))
+            print(f"{W}x{H} frame {k}: tone map {int((d > 0).sum())} of {d.size} bytes differ", end="")
+            assert d.max() <= 1 and (d > 0).sum() <= max(1, d.size // 1000)
+            st.txaa(cur, old, t.ubo)
+            got = st.read_txaa()
+            _, ref8, _ = O.txaa(cur, old, t.ubo, disp, history, precision=32)
+            d = np.abs(got.astype(np.int16) - ref8.astype(np.int16))
+            print(f"; TXAA {int((d > 0).sum())} differ, max {int(d.max())}")
+            assert d.max() <= 1 and (d > 0).sum() <= max(1, d.size // 1000)
+            history = got
+            old = cur
+
+
 def test_post_error_paths():
     with M.CloudStage(32, 16, M.VolumeSpec(32, 16, 32, 64)) as st:
         with pytest.raises(M.MtcError):
