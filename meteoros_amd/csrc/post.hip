@@ -62,7 +62,9 @@ __device__ __forceinline__ float decode(float4 c) {
 // Map.  One lane per map texel: dot(rgba, bitDec) of the mask texel, or of
 // the sampler's opaque black border outside the image.  Decoding is linear,
 // so decoding before filtering equals the shader's filtering before
-// decoding up to FP32 rounding.
+// decoding up to FP32 rounding.  Each map row takes whole workgroups of 256
+// texels, so the row, and the mask row it reads, are uniform: scalar row
+// pointers and 32-bit column offsets, no per-lane division.
 __global__ __launch_bounds__(256) void god_rays_setup_kernel(PostConsts pc, const float4* __restrict__ mask,
                                                              int table_blocks, float* __restrict__ map,
                                                              GodRayEntry* __restrict__ tab) {
@@ -98,10 +100,16 @@ __global__ __launch_bounds__(256) void god_rays_setup_kernel(PostConsts pc, cons
     return;
   }
   const int P = god_rays_map_pitch(W);
-  const long long n = ((long long)blockIdx.x - table_blocks) * 256 + threadIdx.x;
-  if (n >= (long long)god_rays_map_rows(H) * P) return;
-  const int row = (int)(n / P), col = (int)(n - (long long)row * P);
-  map[n] = decode(texel_or_border(mask, W, H, col - 4, row - 1));
+  const int bpr = (P + 255) / 256;  // workgroups per map row
+  const int b = (int)blockIdx.x - table_blocks;
+  const int row = b / bpr;
+  const int col = (b - row * bpr) * 256 + (int)threadIdx.x;
+  if (row >= god_rays_map_rows(H) || col >= P) return;
+  const int y = row - 1, x = col - 4;
+  float4 c = make_float4(0.0f, 0.0f, 0.0f, 1.0f);
+  if (y >= 0 && y < H && x >= 0 && x < W)
+    c = *(const float4*)((const char*)(mask + (size_t)y * W) + (uint32_t)x * 16u);
+  *(float*)((char*)(map + (size_t)row * P) + (uint32_t)col * 4u) = decode(c);
 }
 
 // The radial blur.  Workgroup = 64 columns x kGodRayGroupRows rows, four
@@ -600,7 +608,7 @@ hipError_t launch_god_rays(const PostConsts& pc, const float4* mask, void* scrat
   GodRayEntry* tab = (GodRayEntry*)((char*)scratch + god_rays_map_span(W, H));
   const int groups = (H + kGodRayGroupRows - 1) / kGodRayGroupRows;
   const int table_blocks = (groups + 3) / 4;
-  const long long map_blocks = ((long long)god_rays_map_rows(H) * god_rays_map_pitch(W) + 255) / 256;
+  const long long map_blocks = (long long)god_rays_map_rows(H) * ((god_rays_map_pitch(W) + 255) / 256);
   hipLaunchKernelGGL(mtc::post::god_rays_setup_kernel, dim3((unsigned)(table_blocks + map_blocks)), dim3(256), 0, s, pc,
                      mask, table_blocks, map, tab);
   hipLaunchKernelGGL(mtc::post::god_rays_kernel, dim3((W + 63) / 64, groups), dim3(256), 0, s, pc, (const float*)map,
