@@ -77,12 +77,15 @@ def test_tone_map_matches_oracle(as_float):
         assert np.all(got[..., 3] == 255)
 
 
-@pytest.mark.parametrize("W,H,pitch", [(65, 37, 85.0), (1, 1, 75.0), (130, 17, 60.0), (200, 300, 89.0)])
+@pytest.mark.parametrize("W,H,pitch", [(65, 37, 85.0), (1, 1, 75.0), (130, 17, 60.0), (200, 300, 89.0),
+                                       (2560, 1440, 70.0)])
 def test_god_rays_odd_sizes_match_oracle(W, H, pitch):
     """The blur's workgroups cover 64 columns x 16 rows: partial waves (65,
     130, 200 columns), a 1x1 image, partial row groups (37, 17, 300 rows) and
     the sun inside the image (pitch 75-89 deg puts it at 19-48% of the
-    height) must all match the oracle."""
+    height) must all match the oracle, and so must an image larger than
+    1080p (2560 x 1440: 3 600 workgroups, more than the device holds at
+    once)."""
     st, cam, tm, sun = _frame(W, H, pitch)
     with st:
         rgba, mask = st.read_output()
